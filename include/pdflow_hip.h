@@ -1,0 +1,159 @@
+/* C-ABI of the MI355X PD-Flow library (libpdflow_hip.so).
+ *
+ * This is the boundary the reference's host code crosses: every entry point replaces one
+ * CSF_cuda host method or one "bridge" free function of pdflow_cudalib.h:125-174, keeping
+ * its meaning and its place in the call sequence of scene_flow_impair.cpp:80-168.  The
+ * C++ drop-in class (include/pdflow_cudalib.h) is a thin wrapper over these functions.
+ *
+ * Conventions (the reference returns void and ignores cudaError_t; here):
+ *   - every function returns PDF_OK (0) or a PDF_ERR_* code; pdf_last_error() returns
+ *     the message of the last failure on that context;
+ *   - all planes are FP32, column-major, element (v, u) at v + u*rows (scene_flow_impair.cpp:232);
+ *   - host-pointer functions are blocking, like the reference's cudaMemcpy; device-pointer
+ *     functions are asynchronous on the context's stream;
+ *   - stage functions enqueue kernels on the context's stream and return without waiting.
+ * No torch, HIP or CUDA types appear in these signatures; a stream is passed as void*.
+ */
+#ifndef PDFLOW_HIP_H
+#define PDFLOW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pdflow_fields.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pdf_ctx pdf_ctx;
+
+enum pdf_status {
+    PDF_OK = 0,
+    PDF_ERR_ARG = 1,      /* bad argument (null pointer, size out of range, bad field id) */
+    PDF_ERR_STATE = 2,    /* call out of sequence (e.g. a stage before pdf_allocate_level) */
+    PDF_ERR_HIP = 3,      /* a HIP runtime call failed */
+    PDF_ERR_NOMEM = 4     /* device allocation failed */
+};
+
+/* ---- lifecycle ------------------------------------------------------------------ */
+
+/* CSF_cuda::readParameters + CSF_cuda::allocateDevMemory (pdflow_cudalib.cu:176-193,
+ * :29-72).  rows x cols is the finest solve size, the input frames are
+ * (640/cam_mode) x (480/cam_mode).  All device memory is allocated here, once; the
+ * per-level "allocation" calls below only select sizes. */
+int pdf_create(pdf_ctx **out, unsigned rows, unsigned cols, float lambda_i, float lambda_d,
+               float mu, const float *g_mask25, unsigned ctf_levels, unsigned cam_mode,
+               float fovh, float fovv);
+
+/* CSF_cuda::freeDeviceMemory (pdflow_cudalib.cu:249-288). */
+int pdf_destroy(pdf_ctx *ctx);
+
+/* Runs the context's work on a caller-owned hipStream_t (NULL = the library's own). */
+int pdf_set_stream(pdf_ctx *ctx, void *hip_stream);
+int pdf_synchronize(pdf_ctx *ctx);
+const char *pdf_last_error(const pdf_ctx *ctx);
+/* message of the last pdf_create failure in this process */
+const char *pdf_last_create_error(void);
+
+/* ---- frame ingest and pyramid ------------------------------------------------------ */
+
+/* CSF_cuda::copyNewFrames (pdflow_cudalib.cu:195-222): uploads one frame (host planes
+ * I, Z of (640/cam_mode) x (480/cam_mode)) and makes the previous frame "old". */
+int pdf_copy_new_frames(pdf_ctx *ctx, const float *I, const float *Z);
+/* Same, from device-resident planes (asynchronous device-to-device copy). */
+int pdf_copy_new_frames_device(pdf_ctx *ctx, const float *dI, const float *dZ);
+
+/* GaussianPyramidBridge (pdflow_cudalib.cu:1172-1184): builds `levels` levels of the
+ * "new" pyramid. */
+int pdf_gaussian_pyramid(pdf_ctx *ctx, unsigned levels, unsigned cam_mode);
+
+/* ---- per-level solve stages ---------------------------------------------------------- */
+
+/* CSF_cuda::allocateMemoryNewLevel (pdflow_cudalib.cu:74-162): selects the level size
+ * and pyramid slot; no allocation happens (the planes are preallocated). */
+int pdf_allocate_level(pdf_ctx *ctx, unsigned rows_i, unsigned cols_i, unsigned level,
+                       unsigned level_image);
+/* CSF_cuda::freeLevelVariables (pdflow_cudalib.cu:290-326): ends the level. */
+int pdf_free_level(pdf_ctx *ctx);
+
+int pdf_assign_zeros(pdf_ctx *ctx);        /* AssignZerosBridge    .cu:1186-1189 */
+int pdf_upsample(pdf_ctx *ctx);            /* UpsampleBridge       .cu:1191-1195 */
+int pdf_rij(pdf_ctx *ctx);                 /* RijBridge            .cu:1207-1210 */
+int pdf_image_gradients(pdf_ctx *ctx);     /* ImageGradientsBridge .cu:1197-1200 */
+int pdf_warping(pdf_ctx *ctx);             /* WarpingBridge        .cu:1202-1205 */
+int pdf_mu_and_step_sizes(pdf_ctx *ctx);   /* MuAndStepSizesBridge .cu:1212-1215 */
+int pdf_gradient(pdf_ctx *ctx);            /* GradientBridge       .cu:1232-1235 */
+int pdf_dual_variables(pdf_ctx *ctx);      /* DualVariablesBridge  .cu:1217-1220 */
+int pdf_divergence(pdf_ctx *ctx);          /* DivergenceBridge     .cu:1227-1230 */
+int pdf_primal_variables(pdf_ctx *ctx);    /* PrimalVariablesBridge .cu:1222-1225 */
+
+/* n iterations of the primal-dual loop of scene_flow_impair.cpp:138-144
+ * (Gradient, DualVariables, Divergence, PrimalVariables) as one fused kernel per
+ * iteration; same results as the four bridges.  Step sizes and gradients are
+ * recomputed in registers, so the sigma_*, tau_*, grad* and divp* planes are not
+ * written by this call. */
+int pdf_pd_iterations(pdf_ctx *ctx, unsigned n);
+
+int pdf_filter(pdf_ctx *ctx);              /* FilterBridge         .cu:1237-1241 */
+int pdf_motion_field(pdf_ctx *ctx);        /* MotionFieldBridge    .cu:1243-1246 */
+
+/* ---- results ------------------------------------------------------------------------ */
+
+/* CSF_cuda::copyMotionField (pdflow_cudalib.cu:239-244): rows_i*cols_i values each. */
+int pdf_copy_motion_field(pdf_ctx *ctx, float *dx, float *dy, float *dz);
+/* CSF_cuda::copyAllSolutions (pdflow_cudalib.cu:224-237): flow plus the level's
+ * pyramid planes of both frames. */
+int pdf_copy_all_solutions(pdf_ctx *ctx, float *dx, float *dy, float *dz, float *depth,
+                           float *depth_old, float *colour, float *colour_old, float *xx,
+                           float *xx_old, float *yy, float *yy_old);
+/* device pointers of the current dx, dy, dz planes (valid until the next solve stage) */
+int pdf_motion_field_device(pdf_ctx *ctx, const float **dx, const float **dy,
+                            const float **dz);
+
+/* ---- whole solve ------------------------------------------------------------------- */
+
+/* The coarse-to-fine loop of scene_flow_impair.cpp:95-168 (PD_flow_opencv::
+ * solveSceneFlowGPU) for the current old/new pyramids, with num_max_iter[ctf_levels]
+ * iterations per level, coarse first (NULL = the reference schedule: 100 at the finest
+ * level, 15 fewer per coarser level).  The finest level's flow is left in dx/dy/dz. */
+int pdf_solve(pdf_ctx *ctx, const unsigned *num_max_iter);
+
+/* Same as pdf_solve, but replays a HIP graph captured on first use (one graph per
+ * schedule and old/new parity). */
+int pdf_solve_graph(pdf_ctx *ctx, const unsigned *num_max_iter);
+
+/* The whole pair of scene_flow_impair.cpp:218-277 + :95-168 from device planes:
+ * ingest frame 1, pyramid, ingest frame 2, pyramid, solve. */
+int pdf_solve_pair_device(pdf_ctx *ctx, const float *dI1, const float *dZ1, const float *dI2,
+                          const float *dZ2, const unsigned *num_max_iter, int use_graph);
+
+/* ---- inspection (no reference counterpart: test and tooling access) ----------------- */
+
+/* rows_i*cols_i values of one plane of the current level (blocking) */
+int pdf_get_field(pdf_ctx *ctx, int field, float *host_out);
+int pdf_set_field(pdf_ctx *ctx, int field, const float *host_in);
+/* one pyramid plane; (640/cam_mode >> level) x (480/cam_mode >> level) values */
+int pdf_get_pyramid(pdf_ctx *ctx, int kind, unsigned level, float *host_out);
+int pdf_level_dims(const pdf_ctx *ctx, unsigned *rows_i, unsigned *cols_i);
+/* tanf(0.5f*fovh) as used by the pyramid and motion-field stages */
+float pdf_tan_half_fovh(const pdf_ctx *ctx);
+
+/* Kernel timing of the fused iteration: when enabled, every pdf_pd_iterations launch
+ * is bracketed by HIP events on the context's stream; pdf_kernel_timing returns the
+ * number of timed launches, their total time (ms) and the pixel-iterations they did. */
+int pdf_enable_kernel_timing(pdf_ctx *ctx, int enable);
+int pdf_kernel_timing(pdf_ctx *ctx, uint64_t *launches, double *total_ms,
+                      uint64_t *pixel_iters);
+
+/* ---- device memory helpers for callers without a HIP binding ------------------------ */
+int pdf_device_alloc(void **ptr, size_t bytes);
+int pdf_device_free(void *ptr);
+int pdf_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int pdf_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int pdf_device_count(int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,667 @@
+// C-ABI host layer of libpdflow_hip.so (declarations and contract: include/pdflow_hip.h).
+//
+// One pdf_ctx is one solver, the counterpart of one CSF_cuda object
+// (pdflow_cudalib.h:38-154).  Differences from the reference's host runtime, by design:
+//   - all device memory is one arena allocated at pdf_create; the reference's 60 cudaMalloc
+//     and 57 cudaFree calls per level (.cu:74-162, :290-326) become size selection, which
+//     also makes the whole solve capturable in a HIP graph;
+//   - kernels receive their plane pointers as arguments, so there is no object copy to
+//     and from the device (ObjectToDevice/BridgeBack, .cu:166-172, :1254-1258);
+//   - work runs on the context's own stream; host-pointer copies are blocking;
+//   - every call reports a status instead of ignoring cudaError_t (.cu:36-37).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/pdflow_hip.h"
+#include "pdflow_internal.hpp"
+
+namespace {
+
+// Fields that the fused iteration ping-pongs (read by neighbours within one launch).
+constexpr int kPingPong[] = {PDF_DU_ACC, PDF_DV_ACC, PDF_DW_ACC, PDF_PUU, PDF_PUV,
+                             PDF_PVU,    PDF_PVV,    PDF_PWU,    PDF_PWV};
+constexpr int kNumPingPong = sizeof(kPingPong) / sizeof(kPingPong[0]);
+
+std::string g_create_error;
+
+struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    float *f_end[PDF_NUM_FIELDS];
+    float *alt_end[PDF_NUM_FIELDS];
+    unsigned rows_i = 0, cols_i = 0, level = 0, level_image = 0;
+};
+
+struct GraphKey {
+    std::vector<unsigned> iters;
+    int parity;
+    bool operator<(const GraphKey &o) const
+    {
+        if (parity != o.parity) return parity < o.parity;
+        return iters < o.iters;
+    }
+};
+
+}  // namespace
+
+struct pdf_ctx {
+    unsigned rows = 0, cols = 0, ctf_levels = 0, cam_mode = 0, width = 0, height = 0;
+    unsigned pyr_levels = 0;
+    float lambda_i = 0, lambda_d = 0, mu = 0, fovh = 0, fovv = 0, tan_half = 0;
+    float g_mask[25] = {};
+    unsigned rows_i = 0, cols_i = 0, level = 0, level_image = 0;
+    bool level_set = false;
+    int parity = 0;  // number of old/new swaps mod 2 (selects graph variants)
+
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    char *arena = nullptr;
+    size_t arena_bytes = 0;
+    size_t cap = 0;  // floats per field plane
+
+    float *colour_wf = nullptr, *depth_wf = nullptr;
+    float *pyr[PDF_NUM_PYR_KINDS][PDF_MAX_PYR_LEVELS] = {};
+    float *f[PDF_NUM_FIELDS] = {};
+    float *alt[PDF_NUM_FIELDS] = {};
+
+    std::map<GraphKey, GraphEntry> graphs;
+
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    std::vector<uint64_t> event_pix;
+    uint64_t timed_launches = 0, timed_pix = 0;
+    double timed_ms = 0.0;
+
+    std::string err;
+};
+
+namespace {
+
+int fail(pdf_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(pdf_ctx *c, hipError_t e, const char *what)
+{
+    return fail(c, PDF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(c, call)                                  \
+    do {                                                  \
+        hipError_t e_ = (call);                           \
+        if (e_ != hipSuccess) return hip_fail(c, e_, #call); \
+    } while (0)
+
+pdf::LevelArgs level_args(const pdf_ctx *c)
+{
+    pdf::LevelArgs a;
+    for (int i = 0; i < PDF_NUM_FIELDS; i++) a.f[i] = c->f[i];
+    const unsigned L = c->level_image;
+    a.I = c->pyr[PDF_PYR_COLOUR][L];
+    a.Z = c->pyr[PDF_PYR_DEPTH][L];
+    a.X = c->pyr[PDF_PYR_XX][L];
+    a.Y = c->pyr[PDF_PYR_YY][L];
+    a.Iold = c->pyr[PDF_PYR_COLOUR_OLD][L];
+    a.Zold = c->pyr[PDF_PYR_DEPTH_OLD][L];
+    a.Xold = c->pyr[PDF_PYR_XX_OLD][L];
+    a.Yold = c->pyr[PDF_PYR_YY_OLD][L];
+    a.rows = c->rows_i;
+    a.cols = c->cols_i;
+    a.lambda_i = c->lambda_i;
+    a.lambda_d = c->lambda_d;
+    a.mu = c->mu;
+    a.inv_f = 2.f * c->tan_half / (float)c->cols;  // finest cols at every level (.cu:1152)
+    std::memcpy(a.g, c->g_mask, sizeof(a.g));
+    return a;
+}
+
+int need_level(pdf_ctx *c)
+{
+    if (!c) return PDF_ERR_ARG;
+    if (!c->level_set) return fail(c, PDF_ERR_STATE, "no level selected: call pdf_allocate_level first");
+    return PDF_OK;
+}
+
+int check_launch(pdf_ctx *c, hipError_t e, const char *what)
+{
+    if (e != hipSuccess) return hip_fail(c, e, what);
+    return PDF_OK;
+}
+
+void swap_pingpong(pdf_ctx *c)
+{
+    for (int k = 0; k < kNumPingPong; k++) std::swap(c->f[kPingPong[k]], c->alt[kPingPong[k]]);
+}
+
+void level_dims_for(const pdf_ctx *c, unsigned i, unsigned *rows_i, unsigned *cols_i,
+                    unsigned *level_image)
+{
+    // scene_flow_impair.cpp:105-108
+    const unsigned s = 1u << (c->ctf_levels - (i + 1));
+    unsigned lw = 0;
+    while ((c->cols << (lw + 1)) <= c->width) lw++;  // log2(width/cols)
+    *cols_i = c->cols / s;
+    *rows_i = c->rows / s;
+    *level_image = c->ctf_levels - i + lw - 1;
+}
+
+void default_schedule(unsigned ctf_levels, unsigned *out6)
+{
+    // scene_flow_impair.cpp:46-52
+    for (int i = 5; i >= 0; i--) out6[i] = (i >= (int)ctf_levels - 1) ? 100u : out6[i + 1] - 15u;
+}
+
+int finish_timing(pdf_ctx *c)
+{
+    if (c->events.empty()) return PDF_OK;
+    HIP_TRY(c, hipEventSynchronize(c->events.back().second));
+    for (size_t i = 0; i < c->events.size(); i++) {
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, c->events[i].first, c->events[i].second));
+        c->timed_ms += ms;
+        c->timed_pix += c->event_pix[i];
+        c->timed_launches++;
+        (void)hipEventDestroy(c->events[i].first);
+        (void)hipEventDestroy(c->events[i].second);
+    }
+    c->events.clear();
+    c->event_pix.clear();
+    return PDF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pdf_last_error(const pdf_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char *pdf_last_create_error(void) { return g_create_error.c_str(); }
+
+int pdf_device_count(int *count)
+{
+    if (!count) return PDF_ERR_ARG;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) { *count = 0; return PDF_ERR_HIP; }
+    return PDF_OK;
+}
+
+int pdf_create(pdf_ctx **out, unsigned rows, unsigned cols, float lambda_i, float lambda_d,
+               float mu, const float *g_mask25, unsigned ctf_levels, unsigned cam_mode,
+               float fovh, float fovv)
+{
+    g_create_error.clear();
+    if (!out || !g_mask25) { g_create_error = "null argument"; return PDF_ERR_ARG; }
+    *out = nullptr;
+    if (cam_mode == 0 || 640 % cam_mode || 480 % cam_mode) {
+        g_create_error = "cam_mode must divide 640 and 480";
+        return PDF_ERR_ARG;
+    }
+    const unsigned width = 640 / cam_mode, height = 480 / cam_mode;
+    if (rows == 0 || cols == 0 || rows > height || cols > width || ctf_levels == 0 ||
+        ctf_levels > PDF_MAX_CTF_LEVELS) {
+        g_create_error = "rows/cols/ctf_levels out of range";
+        return PDF_ERR_ARG;
+    }
+    if ((rows >> (ctf_levels - 1)) << (ctf_levels - 1) != rows ||
+        (cols >> (ctf_levels - 1)) << (ctf_levels - 1) != cols) {
+        g_create_error = "rows and cols must halve exactly ctf_levels-1 times";
+        return PDF_ERR_ARG;
+    }
+    // .cu:41: pyr_levels = round(log2(width/cols)) + ctf_levels
+    const unsigned pyr_levels = (unsigned)roundf(log2f((float)(width / cols))) + ctf_levels;
+    if (pyr_levels > PDF_MAX_PYR_LEVELS) { g_create_error = "too many pyramid levels"; return PDF_ERR_ARG; }
+    if ((width >> (pyr_levels - 1)) << (pyr_levels - 1) != width ||
+        (height >> (pyr_levels - 1)) << (pyr_levels - 1) != height) {
+        g_create_error = "input frames must halve exactly for every pyramid level";
+        return PDF_ERR_ARG;
+    }
+
+    pdf_ctx *c = new pdf_ctx();
+    c->rows = rows; c->cols = cols; c->lambda_i = lambda_i; c->lambda_d = lambda_d; c->mu = mu;
+    c->ctf_levels = ctf_levels; c->cam_mode = cam_mode; c->fovh = fovh; c->fovv = fovv;
+    c->width = width; c->height = height; c->pyr_levels = pyr_levels;
+    c->tan_half = tanf(0.5f * fovh);
+    std::memcpy(c->g_mask, g_mask25, sizeof(c->g_mask));
+
+    const size_t P0 = (size_t)width * height;
+    c->cap = P0;
+    auto rnd = [](size_t floats) { return (floats * sizeof(float) + 255) & ~size_t(255); };
+    size_t bytes = 2 * rnd(P0);
+    for (unsigned l = 0; l < pyr_levels; l++) bytes += PDF_NUM_PYR_KINDS * rnd(P0 >> (2 * l));
+    bytes += (PDF_NUM_FIELDS + kNumPingPong) * rnd(c->cap);
+    hipError_t e = hipMalloc((void **)&c->arena, bytes);
+    if (e != hipSuccess) {
+        g_create_error = std::string("hipMalloc of the solver arena failed: ") + hipGetErrorString(e);
+        delete c;
+        return PDF_ERR_NOMEM;
+    }
+    c->arena_bytes = bytes;
+    char *p = c->arena;
+    auto take = [&](size_t floats) { float *r = (float *)p; p += rnd(floats); return r; };
+    c->colour_wf = take(P0);
+    c->depth_wf = take(P0);
+    for (unsigned l = 0; l < pyr_levels; l++)
+        for (int k = 0; k < PDF_NUM_PYR_KINDS; k++) c->pyr[k][l] = take(P0 >> (2 * l));
+    for (int i = 0; i < PDF_NUM_FIELDS; i++) c->f[i] = take(c->cap);
+    for (int k = 0; k < kNumPingPong; k++) c->alt[kPingPong[k]] = take(c->cap);
+
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMemsetAsync(c->arena, 0, bytes, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        g_create_error = std::string("stream set-up failed: ") + hipGetErrorString(e);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        (void)hipFree(c->arena);
+        delete c;
+        return PDF_ERR_HIP;
+    }
+    c->own_stream = true;
+    *out = c;
+    return PDF_OK;
+}
+
+int pdf_destroy(pdf_ctx *c)
+{
+    if (!c) return PDF_ERR_ARG;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
+    for (auto &ev : c->events) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    (void)hipFree(c->arena);
+    delete c;
+    return PDF_OK;
+}
+
+int pdf_set_stream(pdf_ctx *c, void *s)
+{
+    if (!c) return PDF_ERR_ARG;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (s == nullptr) {
+        if (!c->own_stream) {
+            HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            c->own_stream = true;
+        }
+        return PDF_OK;
+    }
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    c->stream = (hipStream_t)s;
+    c->own_stream = false;
+    for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
+    c->graphs.clear();
+    return PDF_OK;
+}
+
+int pdf_synchronize(pdf_ctx *c)
+{
+    if (!c) return PDF_ERR_ARG;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return finish_timing(c);
+}
+
+static void swap_old_new(pdf_ctx *c)
+{
+    // .cu:203-221
+    for (unsigned l = 0; l < PDF_MAX_PYR_LEVELS; l++)
+        for (int k = 0; k < PDF_NUM_PYR_KINDS; k += 2) std::swap(c->pyr[k][l], c->pyr[k + 1][l]);
+    c->parity ^= 1;
+}
+
+int pdf_copy_new_frames(pdf_ctx *c, const float *I, const float *Z)
+{
+    if (!c || !I || !Z) return fail(c, PDF_ERR_ARG, "null frame pointer");
+    const size_t bytes = (size_t)c->width * c->height * sizeof(float);
+    HIP_TRY(c, hipMemcpyAsync(c->depth_wf, Z, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->colour_wf, I, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    swap_old_new(c);
+    return PDF_OK;
+}
+
+int pdf_copy_new_frames_device(pdf_ctx *c, const float *dI, const float *dZ)
+{
+    if (!c || !dI || !dZ) return fail(c, PDF_ERR_ARG, "null frame pointer");
+    const size_t bytes = (size_t)c->width * c->height * sizeof(float);
+    HIP_TRY(c, hipMemcpyAsync(c->depth_wf, dZ, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->colour_wf, dI, bytes, hipMemcpyDeviceToDevice, c->stream));
+    swap_old_new(c);
+    return PDF_OK;
+}
+
+int pdf_gaussian_pyramid(pdf_ctx *c, unsigned levels, unsigned cam_mode)
+{
+    if (!c) return PDF_ERR_ARG;
+    if (levels > c->pyr_levels || cam_mode != c->cam_mode)
+        return fail(c, PDF_ERR_ARG, "pyramid levels or cam_mode do not match pdf_create");
+    for (unsigned i = 0; i < levels; i++) {
+        pdf::PyrArgs a;
+        // .cu:1176-1177
+        a.cols_i = (unsigned)(640.f / ((float)cam_mode * powf(2.f, (float)i)));
+        a.rows_i = (unsigned)(480.f / ((float)cam_mode * powf(2.f, (float)i)));
+        a.level = i;
+        a.inv_f_i = 2.f * c->tan_half / (float)a.cols_i;
+        a.colp = i ? c->pyr[PDF_PYR_COLOUR][i - 1] : nullptr;
+        a.depp = i ? c->pyr[PDF_PYR_DEPTH][i - 1] : nullptr;
+        a.colour_wf = c->colour_wf;
+        a.depth_wf = c->depth_wf;
+        a.col = c->pyr[PDF_PYR_COLOUR][i];
+        a.dep = c->pyr[PDF_PYR_DEPTH][i];
+        a.xx = c->pyr[PDF_PYR_XX][i];
+        a.yy = c->pyr[PDF_PYR_YY][i];
+        std::memcpy(a.g, c->g_mask, sizeof(a.g));
+        int rc = check_launch(c, pdf::launch_pyramid(a, c->stream), "pyramid kernel");
+        if (rc) return rc;
+    }
+    return PDF_OK;
+}
+
+int pdf_allocate_level(pdf_ctx *c, unsigned rows_i, unsigned cols_i, unsigned level,
+                       unsigned level_image)
+{
+    if (!c) return PDF_ERR_ARG;
+    if (level_image >= c->pyr_levels || rows_i == 0 || cols_i == 0 ||
+        rows_i != (c->height >> level_image) || cols_i != (c->width >> level_image) ||
+        (size_t)rows_i * cols_i > c->cap)
+        return fail(c, PDF_ERR_ARG, "level size does not match the pyramid slot");
+    c->rows_i = rows_i;
+    c->cols_i = cols_i;
+    c->level = level;
+    c->level_image = level_image;
+    c->level_set = true;
+    return PDF_OK;
+}
+
+int pdf_free_level(pdf_ctx *c)
+{
+    if (!c) return PDF_ERR_ARG;
+    return PDF_OK;  // planes stay allocated; dx/dy/dz and the *_l planes remain readable
+}
+
+#define STAGE(name, launcher)                                              \
+    int name(pdf_ctx *c)                                                   \
+    {                                                                      \
+        int rc = need_level(c);                                            \
+        if (rc) return rc;                                                 \
+        return check_launch(c, pdf::launcher(level_args(c), c->stream), #launcher); \
+    }
+
+STAGE(pdf_assign_zeros, launch_assign_zeros)
+STAGE(pdf_rij, launch_rij)
+STAGE(pdf_image_gradients, launch_image_gradients)
+STAGE(pdf_warping, launch_warping)
+STAGE(pdf_mu_and_step_sizes, launch_mu_steps)
+STAGE(pdf_gradient, launch_gradient)
+STAGE(pdf_dual_variables, launch_dual)
+STAGE(pdf_divergence, launch_divergence)
+STAGE(pdf_primal_variables, launch_primal)
+STAGE(pdf_filter, launch_filter)
+STAGE(pdf_motion_field, launch_motion_field)
+
+int pdf_upsample(pdf_ctx *c)
+{
+    int rc = need_level(c);
+    if (rc) return rc;
+    if (c->level == 0 || (c->rows_i & 1) || (c->cols_i & 1))
+        return fail(c, PDF_ERR_STATE, "upsampling needs a solved coarser level and even sizes");
+    return check_launch(c, pdf::launch_upsample(level_args(c), c->stream), "upsample");
+}
+
+int pdf_pd_iterations(pdf_ctx *c, unsigned n)
+{
+    int rc = need_level(c);
+    if (rc) return rc;
+    for (unsigned it = 0; it < n; it++) {
+        pdf::PDArgs a;
+        a.ri = c->f[PDF_RI]; a.rj = c->f[PDF_RJ];
+        a.du_prev = c->f[PDF_DU_PREV]; a.dv_prev = c->f[PDF_DV_PREV];
+        a.mu_uv = c->f[PDF_MU_UV]; a.ddu = c->f[PDF_DDU]; a.ddv = c->f[PDF_DDV];
+        a.ddt = c->f[PDF_DDT]; a.dct = c->f[PDF_DCT]; a.dcu = c->f[PDF_DCU]; a.dcv = c->f[PDF_DCV];
+        a.du = c->f[PDF_DU_NEW]; a.dv = c->f[PDF_DV_NEW]; a.dw = c->f[PDF_DW_NEW]; a.pd = c->f[PDF_PD];
+        a.du_acc = c->f[PDF_DU_ACC]; a.dv_acc = c->f[PDF_DV_ACC]; a.dw_acc = c->f[PDF_DW_ACC];
+        a.puu = c->f[PDF_PUU]; a.puv = c->f[PDF_PUV]; a.pvu = c->f[PDF_PVU];
+        a.pvv = c->f[PDF_PVV]; a.pwu = c->f[PDF_PWU]; a.pwv = c->f[PDF_PWV];
+        a.du_acc_o = c->alt[PDF_DU_ACC]; a.dv_acc_o = c->alt[PDF_DV_ACC]; a.dw_acc_o = c->alt[PDF_DW_ACC];
+        a.puu_o = c->alt[PDF_PUU]; a.puv_o = c->alt[PDF_PUV]; a.pvu_o = c->alt[PDF_PVU];
+        a.pvv_o = c->alt[PDF_PVV]; a.pwu_o = c->alt[PDF_PWU]; a.pwv_o = c->alt[PDF_PWV];
+        a.rows = c->rows_i; a.cols = c->cols_i;
+        a.lambda_i = c->lambda_i; a.lambda_d = c->lambda_d;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (c->timing) {
+            HIP_TRY(c, hipEventCreate(&e0));
+            HIP_TRY(c, hipEventCreate(&e1));
+            HIP_TRY(c, hipEventRecord(e0, c->stream));
+        }
+        rc = check_launch(c, pdf::launch_pd_fused(a, c->stream), "fused primal-dual iteration");
+        if (rc) return rc;
+        if (c->timing) {
+            HIP_TRY(c, hipEventRecord(e1, c->stream));
+            c->events.emplace_back(e0, e1);
+            c->event_pix.push_back((uint64_t)c->rows_i * c->cols_i);
+        }
+        swap_pingpong(c);
+    }
+    return PDF_OK;
+}
+
+int pdf_copy_motion_field(pdf_ctx *c, float *dx, float *dy, float *dz)
+{
+    int rc = need_level(c);
+    if (rc) return rc;
+    if (!dx || !dy || !dz) return fail(c, PDF_ERR_ARG, "null output pointer");
+    const size_t bytes = (size_t)c->rows_i * c->cols_i * sizeof(float);
+    HIP_TRY(c, hipMemcpyAsync(dx, c->f[PDF_DX], bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(dy, c->f[PDF_DY], bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(dz, c->f[PDF_DZ], bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return finish_timing(c);
+}
+
+int pdf_copy_all_solutions(pdf_ctx *c, float *dx, float *dy, float *dz, float *depth,
+                           float *depth_old, float *colour, float *colour_old, float *xx,
+                           float *xx_old, float *yy, float *yy_old)
+{
+    int rc = need_level(c);
+    if (rc) return rc;
+    float *dst[11] = {dx, dy, dz, depth, depth_old, colour, colour_old, xx, xx_old, yy, yy_old};
+    const unsigned L = c->level_image;
+    const float *src[11] = {c->f[PDF_DX], c->f[PDF_DY], c->f[PDF_DZ],
+                            c->pyr[PDF_PYR_DEPTH][L], c->pyr[PDF_PYR_DEPTH_OLD][L],
+                            c->pyr[PDF_PYR_COLOUR][L], c->pyr[PDF_PYR_COLOUR_OLD][L],
+                            c->pyr[PDF_PYR_XX][L], c->pyr[PDF_PYR_XX_OLD][L],
+                            c->pyr[PDF_PYR_YY][L], c->pyr[PDF_PYR_YY_OLD][L]};
+    const size_t bytes = (size_t)c->rows_i * c->cols_i * sizeof(float);
+    for (int k = 0; k < 11; k++) {
+        if (!dst[k]) return fail(c, PDF_ERR_ARG, "null output pointer");
+        HIP_TRY(c, hipMemcpyAsync(dst[k], src[k], bytes, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return finish_timing(c);
+}
+
+int pdf_motion_field_device(pdf_ctx *c, const float **dx, const float **dy, const float **dz)
+{
+    if (!c || !dx || !dy || !dz) return PDF_ERR_ARG;
+    *dx = c->f[PDF_DX];
+    *dy = c->f[PDF_DY];
+    *dz = c->f[PDF_DZ];
+    return PDF_OK;
+}
+
+static int solve_sequence(pdf_ctx *c, const unsigned *iters)
+{
+    // scene_flow_impair.cpp:103-167, one level at a time, coarse to fine
+    for (unsigned i = 0; i < c->ctf_levels; i++) {
+        unsigned rows_i, cols_i, level_image;
+        level_dims_for(c, i, &rows_i, &cols_i, &level_image);
+        int rc = pdf_allocate_level(c, rows_i, cols_i, i, level_image);
+        if (!rc) rc = pdf_assign_zeros(c);
+        if (!rc && i > 0) rc = pdf_upsample(c);
+        if (!rc) rc = pdf_rij(c);
+        if (!rc) rc = pdf_image_gradients(c);
+        if (!rc) rc = pdf_warping(c);
+        if (!rc) rc = pdf_mu_and_step_sizes(c);
+        if (!rc) rc = pdf_pd_iterations(c, iters[i]);
+        if (!rc) rc = pdf_filter(c);
+        if (!rc) rc = pdf_motion_field(c);
+        if (rc) return rc;
+    }
+    return PDF_OK;
+}
+
+static void resolve_schedule(const pdf_ctx *c, const unsigned *in, unsigned *out6)
+{
+    if (in) {
+        for (unsigned i = 0; i < 6; i++) out6[i] = i < c->ctf_levels ? in[i] : 0u;
+    } else {
+        default_schedule(c->ctf_levels, out6);
+    }
+}
+
+int pdf_solve(pdf_ctx *c, const unsigned *num_max_iter)
+{
+    if (!c) return PDF_ERR_ARG;
+    unsigned it[6];
+    resolve_schedule(c, num_max_iter, it);
+    return solve_sequence(c, it);
+}
+
+int pdf_solve_graph(pdf_ctx *c, const unsigned *num_max_iter)
+{
+    if (!c) return PDF_ERR_ARG;
+    if (c->timing) return pdf_solve(c, num_max_iter);  // per-launch events are not captured
+    unsigned it[6];
+    resolve_schedule(c, num_max_iter, it);
+    GraphKey key{std::vector<unsigned>(it, it + c->ctf_levels), c->parity};
+    auto found = c->graphs.find(key);
+    if (found == c->graphs.end()) {
+        // Capture the whole sequence.  The graph bakes in the plane pointers of every
+        // stage; it does not depend on what the ping-pong planes hold when it starts,
+        // because each level begins by zeroing them.  The host-side pointer state the
+        // capture leaves behind is the state every replay ends in; it is stored with the
+        // graph and restored after each launch.
+        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = solve_sequence(c, it);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+        if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+        GraphEntry g;
+        e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
+        std::memcpy(g.f_end, c->f, sizeof(g.f_end));
+        std::memcpy(g.alt_end, c->alt, sizeof(g.alt_end));
+        g.rows_i = c->rows_i; g.cols_i = c->cols_i; g.level = c->level; g.level_image = c->level_image;
+        found = c->graphs.emplace(key, g).first;
+    }
+    const GraphEntry &g = found->second;
+    HIP_TRY(c, hipGraphLaunch(g.exec, c->stream));
+    std::memcpy(c->f, g.f_end, sizeof(g.f_end));
+    std::memcpy(c->alt, g.alt_end, sizeof(g.alt_end));
+    c->rows_i = g.rows_i; c->cols_i = g.cols_i; c->level = g.level; c->level_image = g.level_image;
+    c->level_set = true;
+    return PDF_OK;
+}
+
+int pdf_solve_pair_device(pdf_ctx *c, const float *dI1, const float *dZ1, const float *dI2,
+                          const float *dZ2, const unsigned *num_max_iter, int use_graph)
+{
+    if (!c) return PDF_ERR_ARG;
+    int rc = pdf_copy_new_frames_device(c, dI1, dZ1);
+    if (!rc) rc = pdf_gaussian_pyramid(c, c->pyr_levels, c->cam_mode);
+    if (!rc) rc = pdf_copy_new_frames_device(c, dI2, dZ2);
+    if (!rc) rc = pdf_gaussian_pyramid(c, c->pyr_levels, c->cam_mode);
+    if (!rc) rc = use_graph ? pdf_solve_graph(c, num_max_iter) : pdf_solve(c, num_max_iter);
+    return rc;
+}
+
+int pdf_get_field(pdf_ctx *c, int field, float *host_out)
+{
+    int rc = need_level(c);
+    if (rc) return rc;
+    if (field < 0 || field >= PDF_NUM_FIELDS || !host_out) return fail(c, PDF_ERR_ARG, "bad field id or pointer");
+    HIP_TRY(c, hipMemcpyAsync(host_out, c->f[field], (size_t)c->rows_i * c->cols_i * sizeof(float),
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return PDF_OK;
+}
+
+int pdf_set_field(pdf_ctx *c, int field, const float *host_in)
+{
+    int rc = need_level(c);
+    if (rc) return rc;
+    if (field < 0 || field >= PDF_NUM_FIELDS || !host_in) return fail(c, PDF_ERR_ARG, "bad field id or pointer");
+    HIP_TRY(c, hipMemcpyAsync(c->f[field], host_in, (size_t)c->rows_i * c->cols_i * sizeof(float),
+                              hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return PDF_OK;
+}
+
+int pdf_get_pyramid(pdf_ctx *c, int kind, unsigned level, float *host_out)
+{
+    if (!c) return PDF_ERR_ARG;
+    if (kind < 0 || kind >= PDF_NUM_PYR_KINDS || level >= c->pyr_levels || !host_out)
+        return fail(c, PDF_ERR_ARG, "bad pyramid kind or level");
+    const size_t n = ((size_t)c->width * c->height) >> (2 * level);
+    HIP_TRY(c, hipMemcpyAsync(host_out, c->pyr[kind][level], n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return PDF_OK;
+}
+
+int pdf_level_dims(const pdf_ctx *c, unsigned *rows_i, unsigned *cols_i)
+{
+    if (!c || !rows_i || !cols_i) return PDF_ERR_ARG;
+    *rows_i = c->rows_i;
+    *cols_i = c->cols_i;
+    return PDF_OK;
+}
+
+float pdf_tan_half_fovh(const pdf_ctx *c) { return c ? c->tan_half : 0.f; }
+
+int pdf_enable_kernel_timing(pdf_ctx *c, int enable)
+{
+    if (!c) return PDF_ERR_ARG;
+    int rc = pdf_synchronize(c);
+    if (rc) return rc;
+    c->timing = enable != 0;
+    c->timed_launches = 0;
+    c->timed_pix = 0;
+    c->timed_ms = 0.0;
+    return PDF_OK;
+}
+
+int pdf_kernel_timing(pdf_ctx *c, uint64_t *launches, double *total_ms, uint64_t *pixel_iters)
+{
+    if (!c) return PDF_ERR_ARG;
+    int rc = pdf_synchronize(c);
+    if (rc) return rc;
+    if (launches) *launches = c->timed_launches;
+    if (total_ms) *total_ms = c->timed_ms;
+    if (pixel_iters) *pixel_iters = c->timed_pix;
+    return PDF_OK;
+}
+
+int pdf_device_alloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return PDF_ERR_ARG;
+    return hipMalloc(ptr, bytes) == hipSuccess ? PDF_OK : PDF_ERR_NOMEM;
+}
+
+int pdf_device_free(void *ptr) { return hipFree(ptr) == hipSuccess ? PDF_OK : PDF_ERR_HIP; }
+
+int pdf_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? PDF_OK : PDF_ERR_HIP;
+}
+
+int pdf_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? PDF_OK : PDF_ERR_HIP;
+}
+
+}  // extern "C"

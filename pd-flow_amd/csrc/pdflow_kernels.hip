@@ -1,0 +1,842 @@
+// gfx950 kernels of the PD-Flow plain solve.
+//
+// Per-pixel arithmetic follows the reference's device bodies (pdflow_cudalib.cu:331-1167)
+// operation for operation: FP32, evaluated in the order written there, compiled with
+// -ffp-contract=off, IEEE division and square root.  rhypotf(a,b) is 1/sqrtf(a*a+b*b) (the
+// reference's own comment at .cu:761), powf(x,2) is x*x, fdividef is IEEE division.
+//
+// Launch shapes are this library's own: 256-thread workgroups (four wave64s) over a 1-D
+// pixel range for the setup stages, and 32x8-pixel LDS tiles for the fused primal-dual
+// iteration.  Nothing here uses MFMA: every stage is a memory-bound stencil.
+#include <hip/hip_runtime.h>
+
+#include "pdflow_internal.hpp"
+
+namespace pdf {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ float rhyp(float a, float b) { return 1.f / sqrtf(a * a + b * b); }
+
+__device__ __forceinline__ unsigned pixel_index() { return blockIdx.x * kBlock + threadIdx.x; }
+
+inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
+
+// ------------------------------------------------------------------ K1 pyramid level
+// computePyramidLevel (.cu:331-434).  One thread per output pixel; the 5x5 mask comes in
+// the kernel arguments, so no block barrier is needed.
+__global__ void __launch_bounds__(kBlock) k_pyramid(PyrArgs a)
+{
+    const unsigned index = pixel_index();
+    const unsigned rows_i = a.rows_i, cols_i = a.cols_i;
+    if (index >= rows_i * cols_i) return;
+    const unsigned v = index % rows_i, u = index / rows_i;
+    const float max_depth_dif = 0.1f;
+    float depth;
+    if (a.level == 0) {
+        a.col[index] = a.colour_wf[index];
+        depth = a.depth_wf[index];
+        a.dep[index] = depth;
+    } else {
+        float sumd = 0.f, sumc = 0.f, wd = 0.f, wc = 0.f;
+        const float dcen = a.depp[2 * v + 4 * u * rows_i];
+        const bool inner = (v > 0) && (v < rows_i - 1) && (u > 0) && (u < cols_i - 1);
+#pragma unroll
+        for (int k = -2; k < 3; k++)
+#pragma unroll
+            for (int l = -2; l < 3; l++) {
+                const int vv = 2 * (int)v + k, uu = 2 * (int)u + l;
+                if (!inner && !(vv >= 0 && vv < 2 * (int)rows_i && uu >= 0 && uu < 2 * (int)cols_i))
+                    continue;
+                const unsigned ip = (unsigned)vv + (unsigned)uu * 2 * rows_i;
+                const float g = a.g[12 + k + 5 * l];
+                sumc += g * a.colp[ip];
+                if (!inner) wc += g;
+                const float d = a.depp[ip];
+                if ((d > 0.f) && (fabsf(d - dcen) < max_depth_dif)) {
+                    const float w = g * (max_depth_dif - fabsf(d - dcen));
+                    wd += w;
+                    sumd += w * d;
+                }
+            }
+        depth = (sumd > 0.f) ? sumd / wd : 0.f;
+        a.dep[index] = depth;
+        a.col[index] = inner ? sumc : sumc / wc;
+    }
+    const float disp_u = 0.5f * (float)(cols_i - 1);
+    const float disp_v = 0.5f * (float)(rows_i - 1);
+    a.xx[index] = ((float)u - disp_u) * depth * a.inv_f_i;
+    a.yy[index] = ((float)v - disp_v) * depth * a.inv_f_i;
+}
+
+// ------------------------------------------------------------------ K2 assign zeros
+__global__ void __launch_bounds__(kBlock) k_assign_zeros(LevelArgs a)
+{
+    // .cu:439-455
+    const unsigned index = pixel_index();
+    if (index >= a.rows * a.cols) return;
+    constexpr int ids[] = {PDF_DU_UPSAMP, PDF_DV_UPSAMP, PDF_DW_UPSAMP, PDF_PD_UPSAMP,
+                           PDF_PUU_UPSAMP, PDF_PUV_UPSAMP, PDF_PVU_UPSAMP, PDF_PVV_UPSAMP,
+                           PDF_PWU_UPSAMP, PDF_PWV_UPSAMP, PDF_DU_PREV, PDF_DV_PREV,
+                           PDF_DU_NEW, PDF_DV_NEW, PDF_DW_NEW, PDF_PD, PDF_PUU, PDF_PUV,
+                           PDF_PVU, PDF_PVV, PDF_PWU, PDF_PWV, PDF_DU_ACC, PDF_DV_ACC,
+                           PDF_DW_ACC};
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(ids) / sizeof(ids[0])); k++) a.f[ids[k]][index] = 0.f;
+}
+
+// ------------------------------------------------------------------ K3/K4 upsampling
+__constant__ int kUps[10] = {PDF_DU_UPSAMP, PDF_DV_UPSAMP, PDF_DW_UPSAMP, PDF_PD_UPSAMP,
+                             PDF_PUU_UPSAMP, PDF_PUV_UPSAMP, PDF_PVU_UPSAMP, PDF_PVV_UPSAMP,
+                             PDF_PWU_UPSAMP, PDF_PWV_UPSAMP};
+
+__global__ void __launch_bounds__(kBlock) k_upsample_copy(LevelArgs a)
+{
+    // .cu:460-477: coarse (v,u) -> fine (2v,2u); du, dv doubled (pixel units)
+    const unsigned index = pixel_index();
+    const unsigned rows_i = a.rows;
+    if (index >= rows_i * a.cols / 4) return;
+    const unsigned v = index % (rows_i / 2);
+    const unsigned u = 2 * index / rows_i;
+    const unsigned big = 2 * (v + u * rows_i);
+    a.f[PDF_DU_UPSAMP][big] = 2.f * a.f[PDF_DU_L][index];
+    a.f[PDF_DV_UPSAMP][big] = 2.f * a.f[PDF_DV_L][index];
+    a.f[PDF_DW_UPSAMP][big] = a.f[PDF_DW_L][index];
+    a.f[PDF_PD_UPSAMP][big] = a.f[PDF_PD_L][index];
+    a.f[PDF_PUU_UPSAMP][big] = a.f[PDF_PUU_L][index];
+    a.f[PDF_PUV_UPSAMP][big] = a.f[PDF_PUV_L][index];
+    a.f[PDF_PVU_UPSAMP][big] = a.f[PDF_PVU_L][index];
+    a.f[PDF_PVV_UPSAMP][big] = a.f[PDF_PVV_L][index];
+    a.f[PDF_PWU_UPSAMP][big] = a.f[PDF_PWU_L][index];
+    a.f[PDF_PWV_UPSAMP][big] = a.f[PDF_PWV_L][index];
+}
+
+__global__ void __launch_bounds__(kBlock) k_upsample_filter(LevelArgs a)
+{
+    // .cu:479-569: 5x5 filter with 4g over the zero-inserted planes; at the border the
+    // weight is 1 - sum of the out-of-range g
+    const unsigned index = pixel_index();
+    const unsigned rows_i = a.rows, cols_i = a.cols;
+    if (index >= rows_i * cols_i) return;
+    const unsigned v = index % rows_i, u = index / rows_i;
+    const float *src[10];
+#pragma unroll
+    for (int q = 0; q < 10; q++) src[q] = a.f[kUps[q]];
+    float acc[10];
+#pragma unroll
+    for (int q = 0; q < 10; q++) acc[q] = 0.f;
+    if ((v > 1) && (v < rows_i - 2) && (u > 1) && (u < cols_i - 2)) {
+#pragma unroll
+        for (int k = -2; k < 3; k++)
+#pragma unroll
+            for (int l = -2; l < 3; l++) {
+                const unsigned ii = v + k + (u + l) * rows_i;
+                const float g = 4.f * a.g[12 + k + 5 * l];
+#pragma unroll
+                for (int q = 0; q < 10; q++) acc[q] += g * src[q][ii];
+            }
+    } else {
+        float aw = 1.f;
+#pragma unroll
+        for (int k = -2; k < 3; k++)
+#pragma unroll
+            for (int l = -2; l < 3; l++) {
+                const int vv = (int)v + k, uu = (int)u + l;
+                const float g = 4.f * a.g[12 + k + 5 * l];
+                if (vv < 0 || vv >= (int)rows_i || uu < 0 || uu >= (int)cols_i) {
+                    aw -= 0.25f * g;
+                    continue;
+                }
+                const unsigned ii = (unsigned)vv + (unsigned)uu * rows_i;
+#pragma unroll
+                for (int q = 0; q < 10; q++) acc[q] += g * src[q][ii];
+            }
+        const float inv = 1.f / aw;
+#pragma unroll
+        for (int q = 0; q < 10; q++) acc[q] *= inv;
+    }
+    a.f[PDF_DU_PREV][index] = acc[0];
+    a.f[PDF_DV_PREV][index] = acc[1];
+    a.f[PDF_DW_NEW][index] = acc[2];
+    a.f[PDF_PD][index] = acc[3];
+    a.f[PDF_PUU][index] = acc[4];
+    a.f[PDF_PUV][index] = acc[5];
+    a.f[PDF_PVU][index] = acc[6];
+    a.f[PDF_PVV][index] = acc[7];
+    a.f[PDF_PWU][index] = acc[8];
+    a.f[PDF_PWV][index] = acc[9];
+    a.f[PDF_DW_ACC][index] = acc[2];
+}
+
+// ------------------------------------------------------------------ K5 Rij
+__global__ void __launch_bounds__(kBlock) k_rij(LevelArgs a)
+{
+    // .cu:725-779
+    const unsigned index = pixel_index();
+    const unsigned rows_i = a.rows, cols_i = a.cols;
+    if (index >= rows_i * cols_i) return;
+    const unsigned v = index % rows_i, u = index / rows_i;
+    float dxu = 0.f, dzu = 0.f, dxu2 = 0.f, dzu2 = 0.f;
+    float dyv = 0.f, dzv = 0.f, dyv2 = 0.f, dzv2 = 0.f;
+    if (u != cols_i - 1) {
+        dxu = a.Xold[index + rows_i] - a.Xold[index];
+        dzu = a.Zold[index + rows_i] - a.Zold[index];
+        dxu2 = a.X[index + rows_i] - a.X[index];
+        dzu2 = a.Z[index + rows_i] - a.Z[index];
+    }
+    if (v != rows_i - 1) {
+        dyv = a.Yold[index + 1] - a.Yold[index];
+        dzv = a.Zold[index + 1] - a.Zold[index];
+        dyv2 = a.Y[index + 1] - a.Y[index];
+        dzv2 = a.Z[index + 1] - a.Z[index];
+    }
+    a.f[PDF_RI][index] = (fabsf(dxu) + fabsf(dzu) > 0.f) ? 2.f * rhyp(dxu, dzu) : 1.f;
+    a.f[PDF_RJ][index] = (fabsf(dyv) + fabsf(dzv) > 0.f) ? 2.f * rhyp(dyv, dzv) : 1.f;
+    a.f[PDF_RI_2][index] = (fabsf(dxu2) + fabsf(dzu2) > 0.f) ? 2.f * rhyp(dxu2, dzu2) : 1.f;
+    a.f[PDF_RJ_2][index] = (fabsf(dyv2) + fabsf(dzv2) > 0.f) ? 2.f * rhyp(dyv2, dzv2) : 1.f;
+}
+
+// ------------------------------------------------------------------ K6 image gradients
+__device__ __forceinline__ float wdiff(float rn, float rp, float fn, float f0, float fp)
+{
+    // r-weighted central difference (.cu:593-595)
+    return (rn * (fn - f0) + rp * (f0 - fp)) / (rn + rp);
+}
+
+__global__ void __launch_bounds__(kBlock) k_image_gradients(LevelArgs a)
+{
+    // .cu:574-627
+    const unsigned index = pixel_index();
+    const unsigned rows_i = a.rows, cols_i = a.cols, r = rows_i;
+    if (index >= rows_i * cols_i) return;
+    const unsigned v = index % rows_i, u = index / rows_i;
+    const float *I = a.I, *Z = a.Z, *ri2 = a.f[PDF_RI_2], *rj2 = a.f[PDF_RJ_2];
+    float dcu, ddu, dcv, ddv;
+    if (u == 0) {
+        dcu = I[index + r] - I[index];
+        ddu = Z[index + r] - Z[index];
+    } else if (u == cols_i - 1) {
+        dcu = I[index] - I[index - r];
+        ddu = Z[index] - Z[index - r];
+    } else {
+        const float rn = ri2[index], rp = ri2[index - r];
+        dcu = wdiff(rn, rp, I[index + r], I[index], I[index - r]);
+        ddu = (Z[index] > 0.f) ? wdiff(rn, rp, Z[index + r], Z[index], Z[index - r]) : 0.f;
+    }
+    if (v == 0) {
+        dcv = I[index + 1] - I[index];
+        ddv = Z[index + 1] - Z[index];
+    } else if (v == rows_i - 1) {
+        dcv = I[index] - I[index - 1];
+        ddv = Z[index] - Z[index - 1];
+    } else {
+        const float rn = rj2[index], rp = rj2[index - 1];
+        dcv = wdiff(rn, rp, I[index + 1], I[index], I[index - 1]);
+        ddv = (Z[index] > 0.f) ? wdiff(rn, rp, Z[index + 1], Z[index], Z[index - 1]) : 0.f;
+    }
+    a.f[PDF_DCU_AUX][index] = dcu;
+    a.f[PDF_DDU_AUX][index] = ddu;
+    a.f[PDF_DCV_AUX][index] = dcv;
+    a.f[PDF_DDV_AUX][index] = ddv;
+}
+
+// ------------------------------------------------------------------ K7 warping
+struct Tap {
+    unsigned su, fu, sv, fv;
+    float iu, iv;
+};
+
+__device__ __forceinline__ Tap make_tap(unsigned rows_i, unsigned cols_i, float iu, float iv)
+{
+    // clamp (.cu:656-659) and the four integer neighbours (.cu:661-664).  A NaN coordinate
+    // is taken as 0 so that no value can produce an index outside the plane (the
+    // reference leaves that case undefined); finite and infinite inputs are unaffected.
+    if (!(iu == iu)) iu = 0.f;
+    if (!(iv == iv)) iv = 0.f;
+    if (iu < 0.f) iu = 0.f;
+    else if (iu > (float)cols_i - 1.f) iu = (float)cols_i - 1.f;
+    if (iv < 0.f) iv = 0.f;
+    else if (iv > (float)rows_i - 1.f) iv = (float)rows_i - 1.f;
+    Tap t;
+    t.su = (unsigned)ceilf(iu);
+    t.fu = (unsigned)floorf(iu);
+    t.sv = (unsigned)ceilf(iv);
+    t.fv = (unsigned)floorf(iv);
+    t.iu = iu;
+    t.iv = iv;
+    return t;
+}
+
+__device__ __forceinline__ float bilinear(const float *mat, unsigned rows_i, const Tap &t)
+{
+    // interpolatePixel (.cu:666-681) after clamping
+    if (t.su == t.fu && t.sv == t.fv) return mat[(unsigned)rintf(t.iv + (float)rows_i * t.iu)];
+    if (t.su == t.fu) {
+        const unsigned cu = (unsigned)rintf(t.iu);
+        return ((float)t.sv - t.iv) * mat[t.fv + rows_i * cu] + (t.iv - (float)t.fv) * mat[t.sv + rows_i * cu];
+    }
+    if (t.sv == t.fv) {
+        const unsigned cv = (unsigned)rintf(t.iv);
+        return ((float)t.su - t.iu) * mat[cv + rows_i * t.fu] + (t.iu - (float)t.fu) * mat[cv + rows_i * t.su];
+    }
+    const float top = ((float)t.su - t.iu) * mat[t.sv + rows_i * t.fu] + (t.iu - (float)t.fu) * mat[t.sv + rows_i * t.su];
+    const float bot = ((float)t.su - t.iu) * mat[t.fv + rows_i * t.fu] + (t.iu - (float)t.fu) * mat[t.fv + rows_i * t.su];
+    return ((float)t.sv - t.iv) * bot + (t.iv - (float)t.fv) * top;
+}
+
+__device__ __forceinline__ float bilinear_depth(const float *mat, unsigned rows_i, const Tap &t)
+{
+    // interpolatePixelDepth (.cu:696-720): nearest neighbour (half to even) if a tap is 0
+    if (mat[t.sv + rows_i * t.su] == 0.f || mat[t.sv + rows_i * t.fu] == 0.f ||
+        mat[t.fv + rows_i * t.su] == 0.f || mat[t.fv + rows_i * t.fu] == 0.f) {
+        const unsigned ru = (unsigned)rintf(t.iu), rv = (unsigned)rintf(t.iv);
+        return mat[rv + rows_i * ru];
+    }
+    return bilinear(mat, rows_i, t);
+}
+
+__global__ void __launch_bounds__(kBlock) k_warping(LevelArgs a)
+{
+    // .cu:629-652
+    const unsigned index = pixel_index();
+    const unsigned rows_i = a.rows, cols_i = a.cols;
+    if (index >= rows_i * cols_i) return;
+    const unsigned v = index % rows_i, u = index / rows_i;
+    const float iu = (float)u + a.f[PDF_DU_PREV][index];
+    const float iv = (float)v + a.f[PDF_DV_PREV][index];
+    const Tap t = make_tap(rows_i, cols_i, iu, iv);
+    a.f[PDF_DCT][index] = bilinear(a.I, rows_i, t) - a.Iold[index];
+    a.f[PDF_DCU][index] = bilinear(a.f[PDF_DCU_AUX], rows_i, t);
+    a.f[PDF_DCV][index] = bilinear(a.f[PDF_DCV_AUX], rows_i, t);
+    const float wd = bilinear_depth(a.Z, rows_i, t);
+    a.f[PDF_DDT][index] = (wd > 0.f) ? wd - a.Zold[index] : 0.f;
+    a.f[PDF_DDU][index] = bilinear(a.f[PDF_DDU_AUX], rows_i, t);
+    a.f[PDF_DDV][index] = bilinear(a.f[PDF_DDV_AUX], rows_i, t);
+}
+
+// ------------------------------------------------------------------ K8 mu and steps
+__device__ __forceinline__ float sigma_pd_of(float mu_uv, float ddu, float ddv)
+{
+    return 1.f / (mu_uv * (1.f + fabsf(ddu) + fabsf(ddv)) + 1e-10f);   // .cu:791
+}
+__device__ __forceinline__ float sigma_p_of(float lam, float r)
+{
+    return 0.5f / (lam * r + 1e-10f);                                    // .cu:792-795
+}
+__device__ __forceinline__ float tau_uv_of(float mu_uv, float dd, float lam, float acu_r)
+{
+    return 1.f / (mu_uv * fabsf(dd) + lam * acu_r + 1e-10f);             // .cu:805-806
+}
+__device__ __forceinline__ float tau_w_of(float mu_uv, float lam, float acu_r)
+{
+    return 1.f / (mu_uv + lam * acu_r + 1e-10f);                         // .cu:807
+}
+
+__global__ void __launch_bounds__(kBlock) k_mu_steps(LevelArgs a)
+{
+    // computeMu + computeStepSizes (.cu:781-808)
+    const unsigned index = pixel_index();
+    const unsigned rows_i = a.rows;
+    if (index >= rows_i * a.cols) return;
+    const float ddu = a.f[PDF_DDU][index], ddv = a.f[PDF_DDV][index], ddt = a.f[PDF_DDT][index];
+    const float mu_uv = a.mu / (1.f + 1000.f * (ddu * ddu + ddv * ddv + ddt * ddt));
+    a.f[PDF_MU_UV][index] = mu_uv;
+    const float ri = a.f[PDF_RI][index], rj = a.f[PDF_RJ][index];
+    a.f[PDF_SIGMA_PD][index] = sigma_pd_of(mu_uv, ddu, ddv);
+    a.f[PDF_SIGMA_PUVX][index] = sigma_p_of(a.lambda_i, ri);
+    a.f[PDF_SIGMA_PUVY][index] = sigma_p_of(a.lambda_i, rj);
+    a.f[PDF_SIGMA_PWX][index] = 0.5f / (ri * a.lambda_d + 1e-10f);
+    a.f[PDF_SIGMA_PWY][index] = 0.5f / (rj * a.lambda_d + 1e-10f);
+    const unsigned v = index % rows_i, u = index / rows_i;
+    float acu_r = ri + rj;
+    if (u > 0) acu_r += a.f[PDF_RI][index - rows_i];
+    if (v > 0) acu_r += a.f[PDF_RJ][index - 1];
+    a.f[PDF_TAU_U][index] = tau_uv_of(mu_uv, ddu, a.lambda_i, acu_r);
+    a.f[PDF_TAU_V][index] = tau_uv_of(mu_uv, ddv, a.lambda_i, acu_r);
+    a.f[PDF_TAU_W][index] = tau_w_of(mu_uv, a.lambda_d, acu_r);
+}
+
+// ------------------------------------------------------------------ K9-K12 (unfused)
+__global__ void __launch_bounds__(kBlock) k_gradient(LevelArgs a)
+{
+    // .cu:982-1013
+    const unsigned index = pixel_index();
+    const unsigned rows_i = a.rows, cols_i = a.cols;
+    if (index >= rows_i * cols_i) return;
+    const unsigned v = index % rows_i, u = index / rows_i;
+    const float *ua = a.f[PDF_DU_ACC], *va = a.f[PDF_DV_ACC], *wa = a.f[PDF_DW_ACC];
+    const float *up = a.f[PDF_DU_PREV], *vp = a.f[PDF_DV_PREV];
+    const float ri = a.f[PDF_RI][index], rj = a.f[PDF_RJ][index];
+    const float u0 = ua[index] + up[index], v0 = va[index] + vp[index], w0 = wa[index];
+    if (u == cols_i - 1) {
+        a.f[PDF_GRADU1][index] = 0.f; a.f[PDF_GRADV1][index] = 0.f; a.f[PDF_GRADW1][index] = 0.f;
+    } else {
+        const unsigned n = index + rows_i;
+        a.f[PDF_GRADU1][index] = ri * ((ua[n] + up[n]) - u0);
+        a.f[PDF_GRADV1][index] = ri * ((va[n] + vp[n]) - v0);
+        a.f[PDF_GRADW1][index] = ri * (wa[n] - w0);
+    }
+    if (v == rows_i - 1) {
+        a.f[PDF_GRADU2][index] = 0.f; a.f[PDF_GRADV2][index] = 0.f; a.f[PDF_GRADW2][index] = 0.f;
+    } else {
+        const unsigned n = index + 1;
+        a.f[PDF_GRADU2][index] = rj * ((ua[n] + up[n]) - u0);
+        a.f[PDF_GRADV2][index] = rj * ((va[n] + vp[n]) - v0);
+        a.f[PDF_GRADW2][index] = rj * (wa[n] - w0);
+    }
+}
+
+__device__ __forceinline__ void project(float &a, float &b)
+{
+    // unit-disc projection (.cu:849-859)
+    const float m = rhyp(a, b);
+    if (m < 1.f) { a = a * m; b = b * m; }
+}
+
+__device__ __forceinline__ float clamp_pd(float pd)
+{
+    // .cu:837-846
+    if (fabsf(pd) > 1.f) return (pd > 1.f) ? 1.f : -1.f;
+    return pd;
+}
+
+__global__ void __launch_bounds__(kBlock) k_dual(LevelArgs a)
+{
+    // .cu:813-887
+    const unsigned index = pixel_index();
+    if (index >= a.rows * a.cols) return;
+    const float li = a.lambda_i, ld = a.lambda_d;
+    float pd = a.f[PDF_PD][index], puu = a.f[PDF_PUU][index], puv = a.f[PDF_PUV][index];
+    float pvu = a.f[PDF_PVU][index], pvv = a.f[PDF_PVV][index];
+    float pwu = a.f[PDF_PWU][index], pwv = a.f[PDF_PWV][index];
+    pd += a.f[PDF_SIGMA_PD][index] * a.f[PDF_MU_UV][index] *
+          (-a.f[PDF_DW_ACC][index] + a.f[PDF_DDT][index] + a.f[PDF_DDU][index] * a.f[PDF_DU_ACC][index] +
+           a.f[PDF_DDV][index] * a.f[PDF_DV_ACC][index]);
+    puu += a.f[PDF_SIGMA_PUVX][index] * li * a.f[PDF_GRADU1][index];
+    puv += a.f[PDF_SIGMA_PUVY][index] * li * a.f[PDF_GRADU2][index];
+    pvu += a.f[PDF_SIGMA_PUVX][index] * li * a.f[PDF_GRADV1][index];
+    pvv += a.f[PDF_SIGMA_PUVY][index] * li * a.f[PDF_GRADV2][index];
+    pwu += a.f[PDF_SIGMA_PWX][index] * ld * a.f[PDF_GRADW1][index];
+    pwv += a.f[PDF_SIGMA_PWY][index] * ld * a.f[PDF_GRADW2][index];
+    a.f[PDF_PD][index] = clamp_pd(pd);
+    project(puu, puv);
+    project(pvu, pvv);
+    project(pwu, pwv);
+    a.f[PDF_PUU][index] = puu; a.f[PDF_PUV][index] = puv;
+    a.f[PDF_PVU][index] = pvu; a.f[PDF_PVV][index] = pvv;
+    a.f[PDF_PWU][index] = pwu; a.f[PDF_PWV][index] = pwv;
+}
+
+__device__ __forceinline__ float div_axis(float r0, float p0, float rm, float pm, bool first, bool last)
+{
+    // one axis of .cu:941-979
+    if (first) return r0 * p0;
+    if (last) return -rm * pm;
+    return r0 * p0 - rm * pm;
+}
+
+__global__ void __launch_bounds__(kBlock) k_divergence(LevelArgs a)
+{
+    // .cu:935-980
+    const unsigned index = pixel_index();
+    const unsigned rows_i = a.rows, cols_i = a.cols;
+    if (index >= rows_i * cols_i) return;
+    const unsigned v = index % rows_i, u = index / rows_i;
+    const float *ri = a.f[PDF_RI], *rj = a.f[PDF_RJ];
+    const bool uf = (u == 0), ul = (u == cols_i - 1), vf = (v == 0), vl = (v == rows_i - 1);
+    const unsigned iu = uf ? index : index - rows_i;   // never read when uf
+    const unsigned iv = vf ? index : index - 1;
+    float du = div_axis(ri[index], a.f[PDF_PUU][index], ri[iu], a.f[PDF_PUU][iu], uf, ul);
+    float dv = div_axis(ri[index], a.f[PDF_PVU][index], ri[iu], a.f[PDF_PVU][iu], uf, ul);
+    float dw = div_axis(ri[index], a.f[PDF_PWU][index], ri[iu], a.f[PDF_PWU][iu], uf, ul);
+    du += div_axis(rj[index], a.f[PDF_PUV][index], rj[iv], a.f[PDF_PUV][iv], vf, vl);
+    dv += div_axis(rj[index], a.f[PDF_PVV][index], rj[iv], a.f[PDF_PVV][iv], vf, vl);
+    dw += div_axis(rj[index], a.f[PDF_PWV][index], rj[iv], a.f[PDF_PWV][iv], vf, vl);
+    a.f[PDF_DIVPU][index] = du;
+    a.f[PDF_DIVPV][index] = dv;
+    a.f[PDF_DIVPW][index] = dw;
+}
+
+struct Primal {
+    float du, dv, dw;
+};
+
+__device__ __forceinline__ Primal primal_update(float du, float dv, float dw, float tu, float tv,
+                                                float tw, float muv, float pd, float ddu,
+                                                float ddv, float divu, float divv, float divw,
+                                                float dct, float dcu, float dcv, float li,
+                                                float ld)
+{
+    // .cu:896-923
+    du += -tu * (muv * ddu * pd - li * divu);
+    dv += -tv * (muv * ddv * pd - li * divv);
+    dw += -tw * (-muv * pd - ld * divw);
+    const float of = dct + dcu * du + dcv * dv;
+    const float th = tu * dcu * dcu + tv * dcv * dcv;
+    if (of < -th) {
+        du += tu * dcu;
+        dv += tv * dcv;
+    } else if (of > th) {
+        du -= tu * dcu;
+        dv -= tv * dcv;
+    } else {
+        const float den = tu * dcu * dcu + tv * dcv * dcv + 1e-10f;
+        du -= tu * dcu * of / den;
+        dv -= tv * dcv * of / den;
+    }
+    return {du, dv, dw};
+}
+
+__global__ void __launch_bounds__(kBlock) k_primal(LevelArgs a)
+{
+    // .cu:889-933
+    const unsigned index = pixel_index();
+    if (index >= a.rows * a.cols) return;
+    const float du0 = a.f[PDF_DU_NEW][index], dv0 = a.f[PDF_DV_NEW][index], dw0 = a.f[PDF_DW_NEW][index];
+    const Primal p = primal_update(du0, dv0, dw0, a.f[PDF_TAU_U][index], a.f[PDF_TAU_V][index],
+                                   a.f[PDF_TAU_W][index], a.f[PDF_MU_UV][index], a.f[PDF_PD][index],
+                                   a.f[PDF_DDU][index], a.f[PDF_DDV][index], a.f[PDF_DIVPU][index],
+                                   a.f[PDF_DIVPV][index], a.f[PDF_DIVPW][index], a.f[PDF_DCT][index],
+                                   a.f[PDF_DCU][index], a.f[PDF_DCV][index], a.lambda_i, a.lambda_d);
+    a.f[PDF_DU_ACC][index] = 2.f * p.du - du0;
+    a.f[PDF_DV_ACC][index] = 2.f * p.dv - dv0;
+    a.f[PDF_DW_ACC][index] = 2.f * p.dw - dw0;
+    a.f[PDF_DU_NEW][index] = p.du;
+    a.f[PDF_DV_NEW][index] = p.dv;
+    a.f[PDF_DW_NEW][index] = p.dw;
+}
+
+// ------------------------------------------------------------------ fused K9-K12
+// One launch = one full primal-dual iteration on a 32x8 tile (256 threads, one pixel
+// each).  The dual step needs (acc+prev) at +1 in v and u; the primal step needs the new
+// duals at -1 in v and u.  The tile therefore stages (acc+prev) on a 1-pixel ring, and
+// recomputes the new TV duals of its low-side halo (one row and one column) instead of
+// synchronising with the neighbouring tiles.  Duals and acc are ping-ponged so halo reads
+// always see the previous iteration.  Step sizes are recomputed from ri/rj/mu/ddu/ddv,
+// bit-identical to the values K8 stores.
+constexpr int TV = 32, TU = 8;                 // tile rows (v) x cols (u)
+constexpr int RV = TV + 2, RU = TU + 2;        // ring region of (acc+prev)
+constexpr int HV = TV + 1, HU = TU + 1;        // tile + low halo (ri, rj, new duals)
+
+struct TVDual {
+    float puu, puv, pvu, pvv, pwu, pwv;
+};
+
+__global__ void __launch_bounds__(TV * TU) k_pd_fused(PDArgs a)
+{
+    __shared__ float sU[RU * RV], sV[RU * RV], sW[RU * RV];
+    __shared__ float sRi[HU * HV], sRj[HU * HV];
+    __shared__ float sP[6][HU * HV];
+
+    const int tid = threadIdx.x;
+    const int tv = tid % TV, tu = tid / TV;
+    const int v0 = blockIdx.x * TV, u0 = blockIdx.y * TU;
+    const int rows = (int)a.rows, cols = (int)a.cols;
+    const float li = a.lambda_i, ld = a.lambda_d;
+
+    // stage (acc + prev) on the ring region [v0-1, v0+TV] x [u0-1, u0+TU]
+    for (int e = tid; e < RU * RV; e += TV * TU) {
+        const int ev = e % RV - 1, eu = e / RV - 1;
+        const int gv = v0 + ev, gu = u0 + eu;
+        float su = 0.f, sv = 0.f, sw = 0.f;
+        if (gv >= 0 && gv < rows && gu >= 0 && gu < cols) {
+            const unsigned g = (unsigned)gv + (unsigned)gu * a.rows;
+            su = a.du_acc[g] + a.du_prev[g];
+            sv = a.dv_acc[g] + a.dv_prev[g];
+            sw = a.dw_acc[g];
+        }
+        sU[e] = su; sV[e] = sv; sW[e] = sw;
+    }
+    // stage ri, rj on tile + low halo [v0-1, v0+TV-1] x [u0-1, u0+TU-1]
+    for (int e = tid; e < HU * HV; e += TV * TU) {
+        const int ev = e % HV - 1, eu = e / HV - 1;
+        const int gv = v0 + ev, gu = u0 + eu;
+        float ri = 0.f, rj = 0.f;
+        if (gv >= 0 && gv < rows && gu >= 0 && gu < cols) {
+            const unsigned g = (unsigned)gv + (unsigned)gu * a.rows;
+            ri = a.ri[g];
+            rj = a.rj[g];
+        }
+        sRi[e] = ri; sRj[e] = rj;
+    }
+    __syncthreads();
+
+    // dual step of the TV duals at a pixel of the tile or its low halo (local ev, eu)
+    auto tv_dual = [&](int ev, int eu, unsigned g, int gv, int gu) -> TVDual {
+        const int r = (eu + 1) * RV + (ev + 1);          // ring index
+        const int h = (eu + 1) * HV + (ev + 1);          // halo-grid index
+        const float ri = sRi[h], rj = sRj[h];
+        const float u_c = sU[r], v_c = sV[r], w_c = sW[r];
+        float g1u = 0.f, g1v = 0.f, g1w = 0.f, g2u = 0.f, g2v = 0.f, g2w = 0.f;
+        if (gu != cols - 1) {
+            g1u = ri * (sU[r + RV] - u_c);
+            g1v = ri * (sV[r + RV] - v_c);
+            g1w = ri * (sW[r + RV] - w_c);
+        }
+        if (gv != rows - 1) {
+            g2u = rj * (sU[r + 1] - u_c);
+            g2v = rj * (sV[r + 1] - v_c);
+            g2w = rj * (sW[r + 1] - w_c);
+        }
+        const float sx = sigma_p_of(li, ri), sy = sigma_p_of(li, rj);
+        const float swx = 0.5f / (ri * ld + 1e-10f), swy = 0.5f / (rj * ld + 1e-10f);
+        TVDual p;
+        p.puu = a.puu[g] + sx * li * g1u;
+        p.puv = a.puv[g] + sy * li * g2u;
+        p.pvu = a.pvu[g] + sx * li * g1v;
+        p.pvv = a.pvv[g] + sy * li * g2v;
+        p.pwu = a.pwu[g] + swx * ld * g1w;
+        p.pwv = a.pwv[g] + swy * ld * g2w;
+        project(p.puu, p.puv);
+        project(p.pvu, p.pvv);
+        project(p.pwu, p.pwv);
+        return p;
+    };
+    auto store_dual = [&](int h, const TVDual &p) {
+        sP[0][h] = p.puu; sP[1][h] = p.puv; sP[2][h] = p.pvu;
+        sP[3][h] = p.pvv; sP[4][h] = p.pwu; sP[5][h] = p.pwv;
+    };
+
+    const int gv = v0 + tv, gu = u0 + tu;
+    const bool inside = gv < rows && gu < cols;
+    const unsigned g = (unsigned)gv + (unsigned)gu * a.rows;
+    const int h = (tu + 1) * HV + (tv + 1);
+    float pd = 0.f;
+    if (inside) {
+        const TVDual p = tv_dual(tv, tu, g, gv, gu);
+        store_dual(h, p);
+        a.puu_o[g] = p.puu; a.puv_o[g] = p.puv; a.pvu_o[g] = p.pvu;
+        a.pvv_o[g] = p.pvv; a.pwu_o[g] = p.pwu; a.pwv_o[g] = p.pwv;
+        const float mu_uv = a.mu_uv[g], ddu = a.ddu[g], ddv = a.ddv[g];
+        pd = a.pd[g] + sigma_pd_of(mu_uv, ddu, ddv) * mu_uv *
+                           (-a.dw_acc[g] + a.ddt[g] + ddu * a.du_acc[g] + ddv * a.dv_acc[g]);
+        pd = clamp_pd(pd);
+        a.pd[g] = pd;
+    }
+    // low halo: row v0-1 (TU pixels) and column u0-1 (TV pixels)
+    if (tid < TU + TV) {
+        int ev, eu;
+        if (tid < TU) { ev = -1; eu = tid; }
+        else { ev = tid - TU; eu = -1; }
+        const int hv = v0 + ev, hu = u0 + eu;
+        if (hv >= 0 && hu >= 0 && hv < rows && hu < cols) {
+            const unsigned hg = (unsigned)hv + (unsigned)hu * a.rows;
+            store_dual((eu + 1) * HV + (ev + 1), tv_dual(ev, eu, hg, hv, hu));
+        }
+    }
+    __syncthreads();
+
+    if (inside) {
+        const float ri = sRi[h], rj = sRj[h];
+        const float riu = sRi[h - HV], rjv = sRj[h - 1];   // (v, u-1) and (v-1, u)
+        const bool uf = gu == 0, ul = gu == cols - 1, vf = gv == 0, vl = gv == rows - 1;
+        float divu = div_axis(ri, sP[0][h], riu, sP[0][h - HV], uf, ul);
+        float divv = div_axis(ri, sP[2][h], riu, sP[2][h - HV], uf, ul);
+        float divw = div_axis(ri, sP[4][h], riu, sP[4][h - HV], uf, ul);
+        divu += div_axis(rj, sP[1][h], rjv, sP[1][h - 1], vf, vl);
+        divv += div_axis(rj, sP[3][h], rjv, sP[3][h - 1], vf, vl);
+        divw += div_axis(rj, sP[5][h], rjv, sP[5][h - 1], vf, vl);
+        float acu_r = ri + rj;
+        if (gu > 0) acu_r += riu;
+        if (gv > 0) acu_r += rjv;
+        const float mu_uv = a.mu_uv[g], ddu = a.ddu[g], ddv = a.ddv[g];
+        const float du0 = a.du[g], dv0 = a.dv[g], dw0 = a.dw[g];
+        const Primal p = primal_update(du0, dv0, dw0, tau_uv_of(mu_uv, ddu, li, acu_r),
+                                       tau_uv_of(mu_uv, ddv, li, acu_r), tau_w_of(mu_uv, ld, acu_r),
+                                       mu_uv, pd, ddu, ddv, divu, divv, divw, a.dct[g], a.dcu[g],
+                                       a.dcv[g], li, ld);
+        a.du_acc_o[g] = 2.f * p.du - du0;
+        a.dv_acc_o[g] = 2.f * p.dv - dv0;
+        a.dw_acc_o[g] = 2.f * p.dw - dw0;
+        a.du[g] = p.du;
+        a.dv[g] = p.dv;
+        a.dw[g] = p.dw;
+    }
+}
+
+// ------------------------------------------------------------------ K13/K14 filter
+__global__ void __launch_bounds__(kBlock) k_saturate(LevelArgs a)
+{
+    // .cu:1018-1038
+    const unsigned index = pixel_index();
+    if (index >= a.rows * a.cols) return;
+    float du = a.f[PDF_DU_NEW][index], dv = a.f[PDF_DV_NEW][index], dw = a.f[PDF_DW_NEW][index];
+    if (du > 1.f) du = 1.f; else if (du < -1.f) du = -1.f;
+    if (dv > 1.f) dv = 1.f; else if (dv < -1.f) dv = -1.f;
+    if (a.Zold[index] == 0.f) dw = 0.f;
+    a.f[PDF_DU_NEW][index] = du + a.f[PDF_DU_PREV][index];
+    a.f[PDF_DV_NEW][index] = dv + a.f[PDF_DV_PREV][index];
+    a.f[PDF_DW_NEW][index] = dw;
+}
+
+// Stable sort of 9 (key, weight) pairs by key.  The reference uses a bubble sort with a
+// strict ">" (.cu:1457-1473); every stable sort yields the same permutation, so this
+// fully unrolled odd-even transposition network (same strict compare, registers only)
+// gives the same output.  Unused slots carry key +inf and weight 0 and stay at the end.
+__device__ __forceinline__ void sort9(float (&k)[9], float (&w)[9])
+{
+#pragma unroll
+    for (int round = 0; round < 9; round++) {
+#pragma unroll
+        for (int i = 1 + (round & 1); i < 9; i += 2) {
+            if (k[i - 1] > k[i]) {
+                const float tk = k[i - 1]; k[i - 1] = k[i]; k[i] = tk;
+                const float tw = w[i - 1]; w[i - 1] = w[i]; w[i] = tw;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ float wmedian_pick(const float (&fld)[9], const float (&cum)[9], float med)
+{
+    // .cu:1103-1110 with register-only indexing
+    unsigned cont = 0;
+    bool go = true;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        go = go && (med > cum[i]);
+        cont += go ? 1u : 0u;
+    }
+    if (cont == 0) return fld[0];
+    float cr = 0.f, cl = 0.f, fr = 0.f, fl = 0.f;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        if ((unsigned)i == cont) { cr = cum[i]; fr = fld[i]; }
+        if ((unsigned)i + 1u == cont) { cl = cum[i]; fl = fld[i]; }
+    }
+    return ((cr - med) * fl + (med - cl) * fr) / (cr - cl);
+}
+
+__global__ void __launch_bounds__(kBlock) k_filter(LevelArgs a)
+{
+    // .cu:1040-1148
+    const unsigned index = pixel_index();
+    const unsigned rows_i = a.rows, cols_i = a.cols;
+    if (index >= rows_i * cols_i) return;
+    const unsigned v = index % rows_i, u = index / rows_i;
+    const float *zo = a.Zold;
+    const float d0 = zo[index];
+    if (d0 > 0.f) {
+        float ku[9], kv[9], kw[9], wu[9], wv[9], ww[9];
+        int n = 0;
+#pragma unroll
+        for (int s = 0; s < 9; s++) {
+            ku[s] = kv[s] = kw[s] = __builtin_inff();
+            wu[s] = wv[s] = ww[s] = 0.f;
+        }
+        // taps in the reference's order: k (v offset) outer, l (u offset) inner
+#pragma unroll
+        for (int k = -1; k < 2; k++)
+#pragma unroll
+            for (int l = -1; l < 2; l++) {
+                const int r = (int)v + k, c = (int)u + l;
+                const bool ok = !(r < 0 || r >= (int)rows_i || c < 0 || c >= (int)cols_i);
+                const unsigned il = ok ? index + l * (int)rows_i + k : index;
+                const float dz = d0 - zo[il], dt = a.f[PDF_DDT][il];
+                const float pres = 1.f / (1.f + 5.f * (dz * dz) + 10.f * (dt * dt));
+                const float fu = a.f[PDF_DU_NEW][il], fv = a.f[PDF_DV_NEW][il], fw = a.f[PDF_DW_NEW][il];
+                // compacted write position n (taps out of range are skipped)
+#pragma unroll
+                for (int s = 0; s < 9; s++)
+                    if (ok && s == n) {
+                        ku[s] = fu; kv[s] = fv; kw[s] = fw;
+                        wu[s] = pres; wv[s] = pres; ww[s] = pres;
+                    }
+                n += ok ? 1 : 0;
+            }
+        sort9(ku, wu);
+        sort9(kv, wv);
+        sort9(kw, ww);
+        float cu[9], cv[9], cw[9];
+        cu[0] = wu[0]; cv[0] = wv[0]; cw[0] = ww[0];
+#pragma unroll
+        for (int i = 1; i < 9; i++) {
+            cu[i] = cu[i - 1] + wu[i];
+            cv[i] = cv[i - 1] + wv[i];
+            cw[i] = cw[i - 1] + ww[i];
+        }
+        // total of u's weights = cu[n-1]; adding the zero weights of unused slots keeps it
+        const float med = 0.5f * cu[8];
+        a.f[PDF_DU_L][index] = wmedian_pick(ku, cu, med);
+        a.f[PDF_DV_L][index] = wmedian_pick(kv, cv, med);
+        a.f[PDF_DW_L][index] = wmedian_pick(kw, cw, med);
+    } else {
+        a.f[PDF_DU_L][index] = a.f[PDF_DU_NEW][index];
+        a.f[PDF_DV_L][index] = a.f[PDF_DV_NEW][index];
+        a.f[PDF_DW_L][index] = a.f[PDF_DW_NEW][index];
+    }
+    a.f[PDF_PD_L][index] = a.f[PDF_PD][index];
+    a.f[PDF_PUU_L][index] = a.f[PDF_PUU][index];
+    a.f[PDF_PUV_L][index] = a.f[PDF_PUV][index];
+    a.f[PDF_PVU_L][index] = a.f[PDF_PVU][index];
+    a.f[PDF_PVV_L][index] = a.f[PDF_PVV][index];
+    a.f[PDF_PWU_L][index] = a.f[PDF_PWU][index];
+    a.f[PDF_PWV_L][index] = a.f[PDF_PWV][index];
+}
+
+// ------------------------------------------------------------------ K15 motion field
+__global__ void __launch_bounds__(kBlock) k_motion_field(LevelArgs a)
+{
+    // .cu:1150-1167
+    const unsigned index = pixel_index();
+    if (index >= a.rows * a.cols) return;
+    const float z = a.Zold[index];
+    if (z > 0.f) {
+        const float dul = a.f[PDF_DU_L][index], dvl = a.f[PDF_DV_L][index], dwl = a.f[PDF_DW_L][index];
+        a.f[PDF_DX][index] = dwl;
+        a.f[PDF_DY][index] = z * dul * a.inv_f + dwl * a.Xold[index] / z;
+        a.f[PDF_DZ][index] = z * dvl * a.inv_f + dwl * a.Yold[index] / z;
+    } else {
+        a.f[PDF_DX][index] = 0.f;
+        a.f[PDF_DY][index] = 0.f;
+        a.f[PDF_DZ][index] = 0.f;
+    }
+}
+
+}  // namespace
+
+#define PDF_LAUNCH(kernel, n, args, stream)                                  \
+    do {                                                                     \
+        if ((n) == 0) return hipSuccess;                                     \
+        hipLaunchKernelGGL(kernel, grid_for(n), dim3(kBlock), 0, stream, args); \
+        return hipGetLastError();                                            \
+    } while (0)
+
+hipError_t launch_pyramid(const PyrArgs &a, hipStream_t s) { PDF_LAUNCH(k_pyramid, a.rows_i * a.cols_i, a, s); }
+hipError_t launch_assign_zeros(const LevelArgs &a, hipStream_t s) { PDF_LAUNCH(k_assign_zeros, a.rows * a.cols, a, s); }
+hipError_t launch_upsample(const LevelArgs &a, hipStream_t s)
+{
+    const unsigned n = a.rows * a.cols;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_upsample_copy, grid_for(n / 4 ? n / 4 : 1), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_upsample_filter, grid_for(n), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_rij(const LevelArgs &a, hipStream_t s) { PDF_LAUNCH(k_rij, a.rows * a.cols, a, s); }
+hipError_t launch_image_gradients(const LevelArgs &a, hipStream_t s) { PDF_LAUNCH(k_image_gradients, a.rows * a.cols, a, s); }
+hipError_t launch_warping(const LevelArgs &a, hipStream_t s) { PDF_LAUNCH(k_warping, a.rows * a.cols, a, s); }
+hipError_t launch_mu_steps(const LevelArgs &a, hipStream_t s) { PDF_LAUNCH(k_mu_steps, a.rows * a.cols, a, s); }
+hipError_t launch_gradient(const LevelArgs &a, hipStream_t s) { PDF_LAUNCH(k_gradient, a.rows * a.cols, a, s); }
+hipError_t launch_dual(const LevelArgs &a, hipStream_t s) { PDF_LAUNCH(k_dual, a.rows * a.cols, a, s); }
+hipError_t launch_divergence(const LevelArgs &a, hipStream_t s) { PDF_LAUNCH(k_divergence, a.rows * a.cols, a, s); }
+hipError_t launch_primal(const LevelArgs &a, hipStream_t s) { PDF_LAUNCH(k_primal, a.rows * a.cols, a, s); }
+hipError_t launch_filter(const LevelArgs &a, hipStream_t s)
+{
+    const unsigned n = a.rows * a.cols;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_saturate, grid_for(n), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_filter, grid_for(n), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_motion_field(const LevelArgs &a, hipStream_t s) { PDF_LAUNCH(k_motion_field, a.rows * a.cols, a, s); }
+
+hipError_t launch_pd_fused(const PDArgs &a, hipStream_t s)
+{
+    if (a.rows == 0 || a.cols == 0) return hipSuccess;
+    const dim3 grid((a.rows + TV - 1) / TV, (a.cols + TU - 1) / TU);
+    hipLaunchKernelGGL(k_pd_fused, grid, dim3(TV * TU), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace pdf

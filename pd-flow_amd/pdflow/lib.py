@@ -1,0 +1,276 @@
+"""ctypes binding of libpdflow_hip.so (C-ABI: include/pdflow_hip.h).
+
+``Solver`` is one solver context, the counterpart of one reference ``CSF_cuda`` object
+(pdflow_cudalib.h:38-154); its methods carry the reference names of the bridges and host
+methods they replace.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import schedule
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+# plane ids, in the order of include/pdflow_fields.h
+_FIELD_NAMES = [
+    "dct", "dcu", "dcv", "ddt", "ddu", "ddv",
+    "dcu_aux", "dcv_aux", "ddu_aux", "ddv_aux",
+    "gradu1", "gradu2", "gradv1", "gradv2", "gradw1", "gradw2",
+    "divpu", "divpv", "divpw",
+    "sigma_pd", "sigma_puvx", "sigma_puvy", "sigma_pwx", "sigma_pwy",
+    "tau_u", "tau_v", "tau_w",
+    "mu_uv", "ri", "rj", "ri_2", "rj_2",
+    "du_acc", "dv_acc", "dw_acc", "du_prev", "dv_prev",
+    "du_upsamp", "dv_upsamp", "dw_upsamp", "pd_upsamp",
+    "puu_upsamp", "puv_upsamp", "pvu_upsamp", "pvv_upsamp", "pwu_upsamp", "pwv_upsamp",
+    "du_new", "dv_new", "dw_new", "pd", "puu", "puv", "pvu", "pvv", "pwu", "pwv",
+    "dx", "dy", "dz",
+    "du_l", "dv_l", "dw_l", "pd_l", "puu_l", "puv_l", "pvu_l", "pvv_l", "pwu_l", "pwv_l",
+]
+FIELDS = {n: i for i, n in enumerate(_FIELD_NAMES)}
+PYR_KINDS = {n: i for i, n in enumerate(
+    ["colour", "colour_old", "depth", "depth_old", "xx", "xx_old", "yy", "yy_old"])}
+
+STATUS = {0: "ok", 1: "bad argument", 2: "out of sequence", 3: "HIP error", 4: "out of device memory"}
+
+
+class PDFlowError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib_path():
+    return os.path.join(_HERE, "libpdflow_hip.so")
+
+
+def hip_library():
+    """Loads libpdflow_hip.so; raises if it is not built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: the HIP library must be built "
+                           "(__graft_entry__.build()); there is no CPU fallback")
+    lib = ctypes.CDLL(path)
+    c_uint, c_int, c_float = ctypes.c_uint, ctypes.c_int, ctypes.c_float
+    vp = ctypes.c_void_p
+    fp = ctypes.POINTER(ctypes.c_float)
+    up = ctypes.POINTER(ctypes.c_uint)
+    sigs = {
+        "pdf_create": ([ctypes.POINTER(vp), c_uint, c_uint, c_float, c_float, c_float, fp, c_uint,
+                        c_uint, c_float, c_float], c_int),
+        "pdf_destroy": ([vp], c_int),
+        "pdf_set_stream": ([vp, vp], c_int),
+        "pdf_synchronize": ([vp], c_int),
+        "pdf_last_error": ([vp], ctypes.c_char_p),
+        "pdf_last_create_error": ([], ctypes.c_char_p),
+        "pdf_copy_new_frames": ([vp, fp, fp], c_int),
+        "pdf_copy_new_frames_device": ([vp, vp, vp], c_int),
+        "pdf_gaussian_pyramid": ([vp, c_uint, c_uint], c_int),
+        "pdf_allocate_level": ([vp, c_uint, c_uint, c_uint, c_uint], c_int),
+        "pdf_free_level": ([vp], c_int),
+        "pdf_pd_iterations": ([vp, c_uint], c_int),
+        "pdf_copy_motion_field": ([vp, fp, fp, fp], c_int),
+        "pdf_copy_all_solutions": ([vp] + [fp] * 11, c_int),
+        "pdf_motion_field_device": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)], c_int),
+        "pdf_solve": ([vp, up], c_int),
+        "pdf_solve_graph": ([vp, up], c_int),
+        "pdf_solve_pair_device": ([vp, vp, vp, vp, vp, up, c_int], c_int),
+        "pdf_get_field": ([vp, c_int, fp], c_int),
+        "pdf_set_field": ([vp, c_int, fp], c_int),
+        "pdf_get_pyramid": ([vp, c_int, c_uint, fp], c_int),
+        "pdf_level_dims": ([vp, up, up], c_int),
+        "pdf_tan_half_fovh": ([vp], c_float),
+        "pdf_enable_kernel_timing": ([vp, c_int], c_int),
+        "pdf_kernel_timing": ([vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(ctypes.c_uint64)], c_int),
+        "pdf_device_alloc": ([ctypes.POINTER(vp), ctypes.c_size_t], c_int),
+        "pdf_device_free": ([vp], c_int),
+        "pdf_memcpy_h2d": ([vp, vp, ctypes.c_size_t], c_int),
+        "pdf_memcpy_d2h": ([vp, vp, ctypes.c_size_t], c_int),
+        "pdf_device_count": ([ctypes.POINTER(c_int)], c_int),
+    }
+    for stage in ("assign_zeros", "upsample", "rij", "image_gradients", "warping",
+                  "mu_and_step_sizes", "gradient", "dual_variables", "divergence",
+                  "primal_variables", "filter", "motion_field"):
+        sigs["pdf_" + stage] = ([vp], c_int)
+    for name, (args, res) in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    hip_library().pdf_device_count(ctypes.byref(n))
+    return n.value
+
+
+def default_params(rows, cam_mode):
+    """Reference parameters of the pair evaluator for a solve of `rows` rows
+    (scene_flow_impair.cpp:38-69)."""
+    return dict(rows=rows, cols=rows * 320 // 240, lambda_i=schedule.LAMBDA_I,
+                lambda_d=schedule.LAMBDA_D, mu=schedule.MU, g_mask=schedule.gaussian_mask(),
+                ctf_levels=schedule.ctf_levels_for_rows(rows), cam_mode=cam_mode,
+                fovh=schedule.FOVH, fovv=schedule.FOVV)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class DeviceBuffer:
+    """A raw device allocation made through the library (no torch needed)."""
+
+    def __init__(self, nbytes):
+        self.lib = hip_library()
+        self.ptr = ctypes.c_void_p()
+        rc = self.lib.pdf_device_alloc(ctypes.byref(self.ptr), nbytes)
+        if rc:
+            raise PDFlowError(rc, f"device allocation of {nbytes} bytes")
+        self.nbytes = nbytes
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        rc = self.lib.pdf_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes)
+        if rc:
+            raise PDFlowError(rc, "host-to-device copy")
+
+    def download(self, arr):
+        rc = self.lib.pdf_memcpy_d2h(arr.ctypes.data, self.ptr, arr.nbytes)
+        if rc:
+            raise PDFlowError(rc, "device-to-host copy")
+        return arr
+
+    def free(self):
+        if self.ptr:
+            self.lib.pdf_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Solver:
+    """One solver context (one reference CSF_cuda object)."""
+
+    def __init__(self, rows=120, cam_mode=4, **overrides):
+        p = default_params(rows, cam_mode)
+        p.update(overrides)
+        self.p = p
+        self.lib = hip_library()
+        self.width, self.height = 640 // p["cam_mode"], 480 // p["cam_mode"]
+        self.ctx = ctypes.c_void_p()
+        mask = np.ascontiguousarray(p["g_mask"], np.float32)
+        rc = self.lib.pdf_create(ctypes.byref(self.ctx), p["rows"], p["cols"], p["lambda_i"],
+                                 p["lambda_d"], p["mu"], _fptr(mask), p["ctf_levels"],
+                                 p["cam_mode"], p["fovh"], p["fovv"])
+        if rc:
+            raise PDFlowError(rc, self.lib.pdf_last_create_error().decode())
+        self.pyr_levels = int(round(np.log2(self.width // p["cols"]))) + p["ctf_levels"]
+
+    # -- plumbing
+    def _chk(self, rc):
+        if rc:
+            raise PDFlowError(rc, self.lib.pdf_last_error(self.ctx).decode())
+
+    def close(self):
+        if self.ctx:
+            self.lib.pdf_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def tan_half(self):
+        return np.float32(self.lib.pdf_tan_half_fovh(self.ctx))
+
+    # -- reference API, stage by stage
+    def copy_new_frames(self, I, Z):
+        I = np.ascontiguousarray(I, np.float32)
+        Z = np.ascontiguousarray(Z, np.float32)
+        assert I.size == self.width * self.height and Z.size == I.size
+        self._chk(self.lib.pdf_copy_new_frames(self.ctx, _fptr(I), _fptr(Z)))
+
+    def gaussian_pyramid(self, levels=None):
+        self._chk(self.lib.pdf_gaussian_pyramid(self.ctx, levels or self.pyr_levels, self.p["cam_mode"]))
+
+    def allocate_level(self, rows_i, cols_i, level, level_image):
+        self._chk(self.lib.pdf_allocate_level(self.ctx, rows_i, cols_i, level, level_image))
+
+    def stage(self, name):
+        self._chk(getattr(self.lib, "pdf_" + name)(self.ctx))
+
+    def pd_iterations(self, n):
+        self._chk(self.lib.pdf_pd_iterations(self.ctx, n))
+
+    def solve(self, iters=None, graph=False):
+        arr = None if iters is None else (ctypes.c_uint * len(iters))(*iters)
+        fn = self.lib.pdf_solve_graph if graph else self.lib.pdf_solve
+        self._chk(fn(self.ctx, arr))
+
+    def synchronize(self):
+        self._chk(self.lib.pdf_synchronize(self.ctx))
+
+    def level_dims(self):
+        r, c = ctypes.c_uint(), ctypes.c_uint()
+        self._chk(self.lib.pdf_level_dims(self.ctx, ctypes.byref(r), ctypes.byref(c)))
+        return r.value, c.value
+
+    def get_field(self, name):
+        r, c = self.level_dims()
+        out = np.empty(r * c, np.float32)
+        self._chk(self.lib.pdf_get_field(self.ctx, FIELDS[name], _fptr(out)))
+        return out
+
+    def set_field(self, name, values):
+        v = np.ascontiguousarray(values, np.float32)
+        r, c = self.level_dims()
+        assert v.size == r * c
+        self._chk(self.lib.pdf_set_field(self.ctx, FIELDS[name], _fptr(v)))
+
+    def get_pyramid(self, kind, level):
+        n = (self.width * self.height) >> (2 * level)
+        out = np.empty(n, np.float32)
+        self._chk(self.lib.pdf_get_pyramid(self.ctx, PYR_KINDS[kind], level, _fptr(out)))
+        return out
+
+    def motion_field(self):
+        r, c = self.level_dims()
+        dx, dy, dz = (np.empty(r * c, np.float32) for _ in range(3))
+        self._chk(self.lib.pdf_copy_motion_field(self.ctx, _fptr(dx), _fptr(dy), _fptr(dz)))
+        return dx, dy, dz
+
+    def solve_pair(self, I1, Z1, I2, Z2, iters=None, graph=False):
+        """The pair evaluator end to end from host planes: returns (dx, dy, dz)."""
+        self.copy_new_frames(I1, Z1)
+        self.gaussian_pyramid()
+        self.copy_new_frames(I2, Z2)
+        self.gaussian_pyramid()
+        self.solve(iters, graph=graph)
+        return self.motion_field()
+
+    def enable_kernel_timing(self, on=True):
+        self._chk(self.lib.pdf_enable_kernel_timing(self.ctx, 1 if on else 0))
+
+    def kernel_timing(self):
+        n, ms, pix = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_uint64()
+        self._chk(self.lib.pdf_kernel_timing(self.ctx, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(pix)))
+        return n.value, ms.value, pix.value
