@@ -95,6 +95,12 @@ int pdf_primal_variables(pdf_ctx *ctx);    /* PrimalVariablesBridge .cu:1222-122
  * written by this call. */
 int pdf_pd_iterations(pdf_ctx *ctx, unsigned n);
 
+/* Iteration-kernel selection (no reference counterpart; the reference launches four kernels
+ * per iteration): levels of at least min_pixels pixels run k iterations per launch in the
+ * temporally blocked kernel (k in {2,3,4,6,8}); k <= 1 disables it.  Results are identical
+ * for every setting.  Default: k = 4 from 120x160 up. */
+int pdf_set_iteration_kernel(pdf_ctx *ctx, int k, unsigned min_pixels);
+
 int pdf_filter(pdf_ctx *ctx);              /* FilterBridge         .cu:1237-1241 */
 int pdf_motion_field(pdf_ctx *ctx);        /* MotionFieldBridge    .cu:1243-1246 */
 

@@ -23,9 +23,12 @@
 
 namespace {
 
-// Fields that the fused iteration ping-pongs (read by neighbours within one launch).
-constexpr int kPingPong[] = {PDF_DU_ACC, PDF_DV_ACC, PDF_DW_ACC, PDF_PUU, PDF_PUV,
-                             PDF_PVU,    PDF_PVV,    PDF_PWU,    PDF_PWV};
+// The 13 state planes of the iteration, in pdf::PDState order; the fused kernels read one
+// copy and write the other (tiles read their neighbours' halo state within a launch).
+constexpr int kPingPong[] = {PDF_DU_NEW, PDF_DV_NEW, PDF_DW_NEW, PDF_DU_ACC, PDF_DV_ACC,
+                             PDF_DW_ACC, PDF_PD,     PDF_PUU,    PDF_PUV,    PDF_PVU,
+                             PDF_PVV,    PDF_PWU,    PDF_PWV};
+static_assert(sizeof(kPingPong) / sizeof(kPingPong[0]) == pdf::S_NUM, "state plane table");
 constexpr int kNumPingPong = sizeof(kPingPong) / sizeof(kPingPong[0]);
 
 std::string g_create_error;
@@ -70,6 +73,11 @@ struct pdf_ctx {
     float *alt[PDF_NUM_FIELDS] = {};
 
     std::map<GraphKey, GraphEntry> graphs;
+
+    // iteration kernel selection: levels of at least tb_min_pixels pixels run tb_k
+    // iterations per launch (temporal blocking); 0 disables it
+    int tb_k = 4;
+    unsigned tb_min_pixels = 120 * 160;
 
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -411,40 +419,66 @@ int pdf_upsample(pdf_ctx *c)
     return check_launch(c, pdf::launch_upsample(level_args(c), c->stream), "upsample");
 }
 
+static pdf::PDArgs pd_args(const pdf_ctx *c)
+{
+    pdf::PDArgs a;
+    a.ri = c->f[PDF_RI]; a.rj = c->f[PDF_RJ];
+    a.du_prev = c->f[PDF_DU_PREV]; a.dv_prev = c->f[PDF_DV_PREV];
+    a.mu_uv = c->f[PDF_MU_UV]; a.ddu = c->f[PDF_DDU]; a.ddv = c->f[PDF_DDV];
+    a.ddt = c->f[PDF_DDT]; a.dct = c->f[PDF_DCT]; a.dcu = c->f[PDF_DCU]; a.dcv = c->f[PDF_DCV];
+    for (int k = 0; k < pdf::S_NUM; k++) {
+        a.in[k] = c->f[kPingPong[k]];
+        a.out[k] = c->alt[kPingPong[k]];
+    }
+    a.rows = c->rows_i; a.cols = c->cols_i;
+    a.lambda_i = c->lambda_i; a.lambda_d = c->lambda_d;
+    return a;
+}
+
+// one launch doing `k` iterations (k == 1: single-iteration kernel)
+static int pd_launch(pdf_ctx *c, int k)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        HIP_TRY(c, hipEventCreate(&e0));
+        HIP_TRY(c, hipEventCreate(&e1));
+        HIP_TRY(c, hipEventRecord(e0, c->stream));
+    }
+    const pdf::PDArgs a = pd_args(c);
+    int rc = check_launch(c, k == 1 ? pdf::launch_pd_fused(a, c->stream) : pdf::launch_pd_tb(a, k, c->stream),
+                          "fused primal-dual iteration");
+    if (rc) return rc;
+    if (c->timing) {
+        HIP_TRY(c, hipEventRecord(e1, c->stream));
+        c->events.emplace_back(e0, e1);
+        c->event_pix.push_back((uint64_t)c->rows_i * c->cols_i * (uint64_t)k);
+    }
+    swap_pingpong(c);
+    return PDF_OK;
+}
+
 int pdf_pd_iterations(pdf_ctx *c, unsigned n)
 {
     int rc = need_level(c);
     if (rc) return rc;
-    for (unsigned it = 0; it < n; it++) {
-        pdf::PDArgs a;
-        a.ri = c->f[PDF_RI]; a.rj = c->f[PDF_RJ];
-        a.du_prev = c->f[PDF_DU_PREV]; a.dv_prev = c->f[PDF_DV_PREV];
-        a.mu_uv = c->f[PDF_MU_UV]; a.ddu = c->f[PDF_DDU]; a.ddv = c->f[PDF_DDV];
-        a.ddt = c->f[PDF_DDT]; a.dct = c->f[PDF_DCT]; a.dcu = c->f[PDF_DCU]; a.dcv = c->f[PDF_DCV];
-        a.du = c->f[PDF_DU_NEW]; a.dv = c->f[PDF_DV_NEW]; a.dw = c->f[PDF_DW_NEW]; a.pd = c->f[PDF_PD];
-        a.du_acc = c->f[PDF_DU_ACC]; a.dv_acc = c->f[PDF_DV_ACC]; a.dw_acc = c->f[PDF_DW_ACC];
-        a.puu = c->f[PDF_PUU]; a.puv = c->f[PDF_PUV]; a.pvu = c->f[PDF_PVU];
-        a.pvv = c->f[PDF_PVV]; a.pwu = c->f[PDF_PWU]; a.pwv = c->f[PDF_PWV];
-        a.du_acc_o = c->alt[PDF_DU_ACC]; a.dv_acc_o = c->alt[PDF_DV_ACC]; a.dw_acc_o = c->alt[PDF_DW_ACC];
-        a.puu_o = c->alt[PDF_PUU]; a.puv_o = c->alt[PDF_PUV]; a.pvu_o = c->alt[PDF_PVU];
-        a.pvv_o = c->alt[PDF_PVV]; a.pwu_o = c->alt[PDF_PWU]; a.pwv_o = c->alt[PDF_PWV];
-        a.rows = c->rows_i; a.cols = c->cols_i;
-        a.lambda_i = c->lambda_i; a.lambda_d = c->lambda_d;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (c->timing) {
-            HIP_TRY(c, hipEventCreate(&e0));
-            HIP_TRY(c, hipEventCreate(&e1));
-            HIP_TRY(c, hipEventRecord(e0, c->stream));
-        }
-        rc = check_launch(c, pdf::launch_pd_fused(a, c->stream), "fused primal-dual iteration");
-        if (rc) return rc;
-        if (c->timing) {
-            HIP_TRY(c, hipEventRecord(e1, c->stream));
-            c->events.emplace_back(e0, e1);
-            c->event_pix.push_back((uint64_t)c->rows_i * c->cols_i);
-        }
-        swap_pingpong(c);
-    }
+    const bool tb = c->tb_k > 1 && (size_t)c->rows_i * c->cols_i >= c->tb_min_pixels;
+    unsigned done = 0;
+    if (tb)
+        for (; done + (unsigned)c->tb_k <= n; done += c->tb_k)
+            if ((rc = pd_launch(c, c->tb_k))) return rc;
+    for (; done < n; done++)
+        if ((rc = pd_launch(c, 1))) return rc;
+    return PDF_OK;
+}
+
+int pdf_set_iteration_kernel(pdf_ctx *c, int k, unsigned min_pixels)
+{
+    if (!c) return PDF_ERR_ARG;
+    if (k > 1 && !pdf::pd_tb_supported(k)) return fail(c, PDF_ERR_ARG, "unsupported iterations per launch");
+    c->tb_k = k > 1 ? k : 0;
+    c->tb_min_pixels = min_pixels;
+    for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
+    c->graphs.clear();
     return PDF_OK;
 }
 

@@ -28,17 +28,26 @@ struct PyrArgs {
     float g[25];
 };
 
-// Operands of the fused primal-dual iteration.
+// State planes of the primal-dual iteration, in the order of PDArgs::in/out.
+enum PDState {
+    S_DU = 0, S_DV, S_DW, S_DU_ACC, S_DV_ACC, S_DW_ACC, S_PD,
+    S_PUU, S_PUV, S_PVU, S_PVV, S_PWU, S_PWV, S_NUM
+};
+
+// Operands of the fused primal-dual iteration: 11 per-level constant planes, the 13 state
+// planes read (in) and written (out).  in and out never alias, so tiles may read their
+// neighbours' halo state while other tiles write theirs.
 struct PDArgs {
     const float *ri, *rj, *du_prev, *dv_prev, *mu_uv, *ddu, *ddv, *ddt, *dct, *dcu, *dcv;
-    float *du, *dv, *dw, *pd;                               // updated in place
-    const float *du_acc, *dv_acc, *dw_acc;                  // read (ping)
-    const float *puu, *puv, *pvu, *pvv, *pwu, *pwv;
-    float *du_acc_o, *dv_acc_o, *dw_acc_o;                  // written (pong)
-    float *puu_o, *puv_o, *pvu_o, *pvv_o, *pwu_o, *pwv_o;
+    const float *in[S_NUM];
+    float *out[S_NUM];
     unsigned rows, cols;
     float lambda_i, lambda_d;
 };
+
+// Iterations per launch supported by the temporally blocked kernel.
+constexpr int kTBMaxK = 8;
+bool pd_tb_supported(int k);
 
 hipError_t launch_pyramid(const PyrArgs &a, hipStream_t s);
 hipError_t launch_assign_zeros(const LevelArgs &a, hipStream_t s);
@@ -52,6 +61,8 @@ hipError_t launch_dual(const LevelArgs &a, hipStream_t s);
 hipError_t launch_divergence(const LevelArgs &a, hipStream_t s);
 hipError_t launch_primal(const LevelArgs &a, hipStream_t s);
 hipError_t launch_pd_fused(const PDArgs &a, hipStream_t s);
+// k fused iterations in one launch (temporal blocking); k must satisfy pd_tb_supported
+hipError_t launch_pd_tb(const PDArgs &a, int k, hipStream_t s);
 hipError_t launch_filter(const LevelArgs &a, hipStream_t s);
 hipError_t launch_motion_field(const LevelArgs &a, hipStream_t s);
 

@@ -1,0 +1,378 @@
+// The primal-dual iteration (reference loop scene_flow_impair.cpp:138-144: Gradient,
+// DualVariables, Divergence, PrimalVariables; bodies pdflow_cudalib.cu:813-1013) as fused
+// gfx950 kernels.  The memory-bound hot path of the solver.
+//
+//   k_pd_fused   one iteration per launch on 32x8 tiles, one pixel per thread.
+//   k_pd_tb<K>   K iterations per launch (temporal blocking): a 64x32 region per 1024-thread
+//                workgroup keeps all 13 state planes and the 11 constant planes in registers
+//                (two pixels per thread), exchanges neighbour values through LDS, and writes
+//                back only the inner (64-2K)x(32-2K) tile, whose values after K iterations
+//                depend on nothing outside the region.  HBM traffic per pixel-iteration drops
+//                from 148 B to about 148*(64*32)/((64-2K)(32-2K))/K B.
+//
+// Both compute every pixel with the same FP32 operations in the same order as the reference
+// kernels, so they agree bit for bit with the unfused stages.  Step sizes are recomputed from
+// ri/rj/mu_uv/ddu/ddv (pure functions of constants, identical to what K8 stores).
+#include <hip/hip_runtime.h>
+
+#include "pdflow_internal.hpp"
+#include "pdflow_pixel.hpp"
+
+namespace pdf {
+namespace {
+
+// ------------------------------------------------------------------ one iteration per launch
+// The dual step needs (acc+prev) at +1 in v and u; the primal step needs the new duals at -1
+// in v and u.  The tile stages (acc+prev) on a 1-pixel ring and recomputes the new TV duals of
+// its low-side halo (one row, one column) instead of synchronising with neighbouring tiles.
+constexpr int TV = 32, TU = 8;                 // tile rows (v) x cols (u)
+constexpr int RV = TV + 2, RU = TU + 2;        // ring region of (acc+prev)
+constexpr int HV = TV + 1, HU = TU + 1;        // tile + low halo (ri, rj, new duals)
+
+struct TVDual {
+    float puu, puv, pvu, pvv, pwu, pwv;
+};
+
+__global__ void __launch_bounds__(TV * TU) k_pd_fused(PDArgs a)
+{
+    __shared__ float sU[RU * RV], sV[RU * RV], sW[RU * RV];
+    __shared__ float sRi[HU * HV], sRj[HU * HV];
+    __shared__ float sP[6][HU * HV];
+
+    const int tid = threadIdx.x;
+    const int tv = tid % TV, tu = tid / TV;
+    const int v0 = blockIdx.x * TV, u0 = blockIdx.y * TU;
+    const int rows = (int)a.rows, cols = (int)a.cols;
+    const float li = a.lambda_i, ld = a.lambda_d;
+    const float *du_acc = a.in[S_DU_ACC], *dv_acc = a.in[S_DV_ACC], *dw_acc = a.in[S_DW_ACC];
+
+    // own pixel: issue every global load before the first barrier
+    const int gv = v0 + tv, gu = u0 + tu;
+    const bool inside = gv < rows && gu < cols;
+    const unsigned g = inside ? (unsigned)gv + (unsigned)gu * a.rows : 0u;
+    float ua = 0, va = 0, wa = 0, du0 = 0, dv0 = 0, dw0 = 0, pd_in = 0, mu_uv = 0, ddu = 0,
+          ddv = 0, ddt = 0, dct = 0, dcu = 0, dcv = 0;
+    float p_in[6] = {0, 0, 0, 0, 0, 0};
+    if (inside) {
+        ua = du_acc[g]; va = dv_acc[g]; wa = dw_acc[g];
+        du0 = a.in[S_DU][g]; dv0 = a.in[S_DV][g]; dw0 = a.in[S_DW][g]; pd_in = a.in[S_PD][g];
+#pragma unroll
+        for (int k = 0; k < 6; k++) p_in[k] = a.in[S_PUU + k][g];
+        mu_uv = a.mu_uv[g]; ddu = a.ddu[g]; ddv = a.ddv[g]; ddt = a.ddt[g];
+        dct = a.dct[g]; dcu = a.dcu[g]; dcv = a.dcv[g];
+    }
+
+    // stage (acc + prev) on the ring region [v0-1, v0+TV] x [u0-1, u0+TU]
+    for (int e = tid; e < RU * RV; e += TV * TU) {
+        const int ev = e % RV - 1, eu = e / RV - 1;
+        const int qv = v0 + ev, qu = u0 + eu;
+        float su = 0.f, sv = 0.f, sw = 0.f;
+        if (qv >= 0 && qv < rows && qu >= 0 && qu < cols) {
+            const unsigned q = (unsigned)qv + (unsigned)qu * a.rows;
+            su = du_acc[q] + a.du_prev[q];
+            sv = dv_acc[q] + a.dv_prev[q];
+            sw = dw_acc[q];
+        }
+        sU[e] = su; sV[e] = sv; sW[e] = sw;
+    }
+    // stage ri, rj on tile + low halo [v0-1, v0+TV-1] x [u0-1, u0+TU-1]
+    for (int e = tid; e < HU * HV; e += TV * TU) {
+        const int ev = e % HV - 1, eu = e / HV - 1;
+        const int qv = v0 + ev, qu = u0 + eu;
+        float ri = 0.f, rj = 0.f;
+        if (qv >= 0 && qv < rows && qu >= 0 && qu < cols) {
+            const unsigned q = (unsigned)qv + (unsigned)qu * a.rows;
+            ri = a.ri[q];
+            rj = a.rj[q];
+        }
+        sRi[e] = ri; sRj[e] = rj;
+    }
+    // the low-halo pixel this thread recomputes (row v0-1 or column u0-1), if any
+    int hev = 0, heu = 0;
+    bool halo = false;
+    float hp[6] = {0, 0, 0, 0, 0, 0};
+    if (tid < TU + TV) {
+        if (tid < TU) { hev = -1; heu = tid; }
+        else { hev = tid - TU; heu = -1; }
+        const int hv = v0 + hev, hu = u0 + heu;
+        halo = hv >= 0 && hu >= 0 && hv < rows && hu < cols;
+        if (halo) {
+            const unsigned hg = (unsigned)hv + (unsigned)hu * a.rows;
+#pragma unroll
+            for (int k = 0; k < 6; k++) hp[k] = a.in[S_PUU + k][hg];
+        }
+    }
+    __syncthreads();
+
+    // dual step of the TV duals at a pixel of the tile or its low halo (local ev, eu)
+    auto tv_dual = [&](int ev, int eu, int qv, int qu, const float *pin) -> TVDual {
+        const int r = (eu + 1) * RV + (ev + 1);          // ring index
+        const int h = (eu + 1) * HV + (ev + 1);          // halo-grid index
+        const float ri = sRi[h], rj = sRj[h];
+        const float u_c = sU[r], v_c = sV[r], w_c = sW[r];
+        float g1u = 0.f, g1v = 0.f, g1w = 0.f, g2u = 0.f, g2v = 0.f, g2w = 0.f;
+        if (qu != cols - 1) {
+            g1u = ri * (sU[r + RV] - u_c);
+            g1v = ri * (sV[r + RV] - v_c);
+            g1w = ri * (sW[r + RV] - w_c);
+        }
+        if (qv != rows - 1) {
+            g2u = rj * (sU[r + 1] - u_c);
+            g2v = rj * (sV[r + 1] - v_c);
+            g2w = rj * (sW[r + 1] - w_c);
+        }
+        const float sx = sigma_p_of(li, ri), sy = sigma_p_of(li, rj);
+        const float swx = 0.5f / (ri * ld + 1e-10f), swy = 0.5f / (rj * ld + 1e-10f);
+        TVDual p;
+        p.puu = pin[0] + sx * li * g1u;
+        p.puv = pin[1] + sy * li * g2u;
+        p.pvu = pin[2] + sx * li * g1v;
+        p.pvv = pin[3] + sy * li * g2v;
+        p.pwu = pin[4] + swx * ld * g1w;
+        p.pwv = pin[5] + swy * ld * g2w;
+        project(p.puu, p.puv);
+        project(p.pvu, p.pvv);
+        project(p.pwu, p.pwv);
+        return p;
+    };
+    auto store_dual = [&](int h, const TVDual &p) {
+        sP[0][h] = p.puu; sP[1][h] = p.puv; sP[2][h] = p.pvu;
+        sP[3][h] = p.pvv; sP[4][h] = p.pwu; sP[5][h] = p.pwv;
+    };
+
+    const int h = (tu + 1) * HV + (tv + 1);
+    float pd = 0.f;
+    if (inside) {
+        const TVDual p = tv_dual(tv, tu, gv, gu, p_in);
+        store_dual(h, p);
+        a.out[S_PUU][g] = p.puu; a.out[S_PUV][g] = p.puv; a.out[S_PVU][g] = p.pvu;
+        a.out[S_PVV][g] = p.pvv; a.out[S_PWU][g] = p.pwu; a.out[S_PWV][g] = p.pwv;
+        pd = pd_in + sigma_pd_of(mu_uv, ddu, ddv) * mu_uv * (-wa + ddt + ddu * ua + ddv * va);
+        pd = clamp_pd(pd);
+        a.out[S_PD][g] = pd;
+    }
+    if (halo) store_dual((heu + 1) * HV + (hev + 1), tv_dual(hev, heu, v0 + hev, u0 + heu, hp));
+    __syncthreads();
+
+    if (inside) {
+        const float ri = sRi[h], rj = sRj[h];
+        const float riu = sRi[h - HV], rjv = sRj[h - 1];   // (v, u-1) and (v-1, u)
+        const bool uf = gu == 0, ul = gu == cols - 1, vf = gv == 0, vl = gv == rows - 1;
+        float divu = div_axis(ri, sP[0][h], riu, sP[0][h - HV], uf, ul);
+        float divv = div_axis(ri, sP[2][h], riu, sP[2][h - HV], uf, ul);
+        float divw = div_axis(ri, sP[4][h], riu, sP[4][h - HV], uf, ul);
+        divu += div_axis(rj, sP[1][h], rjv, sP[1][h - 1], vf, vl);
+        divv += div_axis(rj, sP[3][h], rjv, sP[3][h - 1], vf, vl);
+        divw += div_axis(rj, sP[5][h], rjv, sP[5][h - 1], vf, vl);
+        float acu_r = ri + rj;
+        if (gu > 0) acu_r += riu;
+        if (gv > 0) acu_r += rjv;
+        const Primal p = primal_update(du0, dv0, dw0, tau_uv_of(mu_uv, ddu, li, acu_r),
+                                       tau_uv_of(mu_uv, ddv, li, acu_r), tau_w_of(mu_uv, ld, acu_r),
+                                       mu_uv, pd, ddu, ddv, divu, divv, divw, dct, dcu, dcv, li, ld);
+        a.out[S_DU_ACC][g] = 2.f * p.du - du0;
+        a.out[S_DV_ACC][g] = 2.f * p.dv - dv0;
+        a.out[S_DW_ACC][g] = 2.f * p.dw - dw0;
+        a.out[S_DU][g] = p.du;
+        a.out[S_DV][g] = p.dv;
+        a.out[S_DW][g] = p.dw;
+    }
+}
+
+// ------------------------------------------------------------------ K iterations per launch
+constexpr int BV = 64, BU = 32;        // region: one wave64 per column pair, 16 waves
+constexpr int BT = 1024;               // threads per workgroup, two pixels (columns) each
+
+struct Px {
+    // state
+    float du, dv, dw, ua, va, wa, pd, p[6];
+    // constants
+    float up, vp, mu, ddu, ddv, ddt, dct, dcu, dcv, ri, rj;
+    // per-launch derived constants (pure functions of the constants)
+    float sxl, syl, swxl, swyl, spdm, tu, tv, tw;
+    int gv, gu;
+    bool on;
+};
+
+template <int K>
+__global__ void __launch_bounds__(BT) k_pd_tb(PDArgs a)
+{
+    constexpr int TVk = BV - 2 * K, TUk = BU - 2 * K;
+    __shared__ float sU[BU * BV], sV[BU * BV], sW[BU * BV];
+    __shared__ float sP[6][BU * BV];
+    __shared__ float sRi[BU * BV], sRj[BU * BV];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rv0 = (int)blockIdx.x * TVk - K, ru0 = (int)blockIdx.y * TUk - K;  // region origin
+    const int rows = (int)a.rows, cols = (int)a.cols;
+    const float li = a.lambda_i, ld = a.lambda_d;
+    const int lv = lane;
+
+    Px x[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        Px &q = x[j];
+        const int lu = 2 * wave + j;
+        q.gv = rv0 + lv;
+        q.gu = ru0 + lu;
+        q.on = q.gv >= 0 && q.gv < rows && q.gu >= 0 && q.gu < cols;
+        const unsigned g = q.on ? (unsigned)q.gv + (unsigned)q.gu * a.rows : 0u;
+        if (q.on) {
+            q.du = a.in[S_DU][g]; q.dv = a.in[S_DV][g]; q.dw = a.in[S_DW][g];
+            q.ua = a.in[S_DU_ACC][g]; q.va = a.in[S_DV_ACC][g]; q.wa = a.in[S_DW_ACC][g];
+            q.pd = a.in[S_PD][g];
+#pragma unroll
+            for (int k = 0; k < 6; k++) q.p[k] = a.in[S_PUU + k][g];
+            q.up = a.du_prev[g]; q.vp = a.dv_prev[g]; q.mu = a.mu_uv[g];
+            q.ddu = a.ddu[g]; q.ddv = a.ddv[g]; q.ddt = a.ddt[g];
+            q.dct = a.dct[g]; q.dcu = a.dcu[g]; q.dcv = a.dcv[g];
+            q.ri = a.ri[g]; q.rj = a.rj[g];
+        } else {
+            q.du = q.dv = q.dw = q.ua = q.va = q.wa = q.pd = 0.f;
+#pragma unroll
+            for (int k = 0; k < 6; k++) q.p[k] = 0.f;
+            q.up = q.vp = q.mu = q.ddu = q.ddv = q.ddt = q.dct = q.dcu = q.dcv = 0.f;
+            q.ri = q.rj = 1.f;
+        }
+        sRi[lu * BV + lv] = q.ri;
+        sRj[lu * BV + lv] = q.rj;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        Px &q = x[j];
+        const int idx = (2 * wave + j) * BV + lv;
+        // first products of the reference expressions (.cu:822-834) are constant per launch
+        q.sxl = sigma_p_of(li, q.ri) * li;
+        q.syl = sigma_p_of(li, q.rj) * li;
+        q.swxl = (0.5f / (q.ri * ld + 1e-10f)) * ld;
+        q.swyl = (0.5f / (q.rj * ld + 1e-10f)) * ld;
+        q.spdm = sigma_pd_of(q.mu, q.ddu, q.ddv) * q.mu;
+        float acu_r = q.ri + q.rj;
+        if (q.gu > 0) acu_r += sRi[idx >= BV ? idx - BV : idx];
+        if (q.gv > 0) acu_r += sRj[lv > 0 ? idx - 1 : idx];
+        q.tu = tau_uv_of(q.mu, q.ddu, li, acu_r);
+        q.tv = tau_uv_of(q.mu, q.ddv, li, acu_r);
+        q.tw = tau_w_of(q.mu, ld, acu_r);
+    }
+
+    for (int it = 0; it < K; it++) {
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int idx = (2 * wave + j) * BV + lv;
+            sU[idx] = x[j].ua + x[j].up;
+            sV[idx] = x[j].va + x[j].vp;
+            sW[idx] = x[j].wa;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            Px &q = x[j];
+            const int lu = 2 * wave + j;
+            const int idx = lu * BV + lv;
+            const float u_c = sU[idx], v_c = sV[idx], w_c = sW[idx];
+            // neighbours outside the region only feed pixels of the discarded band
+            const int iu = lu + 1 < BU ? idx + BV : idx;
+            const int iv = lv + 1 < BV ? idx + 1 : idx;
+            float g1u = 0.f, g1v = 0.f, g1w = 0.f, g2u = 0.f, g2v = 0.f, g2w = 0.f;
+            if (q.gu != cols - 1) {
+                g1u = q.ri * (sU[iu] - u_c);
+                g1v = q.ri * (sV[iu] - v_c);
+                g1w = q.ri * (sW[iu] - w_c);
+            }
+            if (q.gv != rows - 1) {
+                g2u = q.rj * (sU[iv] - u_c);
+                g2v = q.rj * (sV[iv] - v_c);
+                g2w = q.rj * (sW[iv] - w_c);
+            }
+            q.p[0] = q.p[0] + q.sxl * g1u;
+            q.p[1] = q.p[1] + q.syl * g2u;
+            q.p[2] = q.p[2] + q.sxl * g1v;
+            q.p[3] = q.p[3] + q.syl * g2v;
+            q.p[4] = q.p[4] + q.swxl * g1w;
+            q.p[5] = q.p[5] + q.swyl * g2w;
+            project(q.p[0], q.p[1]);
+            project(q.p[2], q.p[3]);
+            project(q.p[4], q.p[5]);
+            q.pd = clamp_pd(q.pd + q.spdm * (-q.wa + q.ddt + q.ddu * q.ua + q.ddv * q.va));
+#pragma unroll
+            for (int k = 0; k < 6; k++) sP[k][idx] = q.p[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            Px &q = x[j];
+            const int lu = 2 * wave + j;
+            const int idx = lu * BV + lv;
+            const int ju = lu > 0 ? idx - BV : idx;
+            const int jv = lv > 0 ? idx - 1 : idx;
+            const float riu = sRi[ju], rjv = sRj[jv];
+            const bool uf = q.gu == 0, ul = q.gu == cols - 1, vf = q.gv == 0, vl = q.gv == rows - 1;
+            float divu = div_axis(q.ri, q.p[0], riu, sP[0][ju], uf, ul);
+            float divv = div_axis(q.ri, q.p[2], riu, sP[2][ju], uf, ul);
+            float divw = div_axis(q.ri, q.p[4], riu, sP[4][ju], uf, ul);
+            divu += div_axis(q.rj, q.p[1], rjv, sP[1][jv], vf, vl);
+            divv += div_axis(q.rj, q.p[3], rjv, sP[3][jv], vf, vl);
+            divw += div_axis(q.rj, q.p[5], rjv, sP[5][jv], vf, vl);
+            const float du0 = q.du, dv0 = q.dv, dw0 = q.dw;
+            const Primal p = primal_update(du0, dv0, dw0, q.tu, q.tv, q.tw, q.mu, q.pd, q.ddu, q.ddv,
+                                           divu, divv, divw, q.dct, q.dcu, q.dcv, li, ld);
+            q.ua = 2.f * p.du - du0;
+            q.va = 2.f * p.dv - dv0;
+            q.wa = 2.f * p.dw - dw0;
+            q.du = p.du;
+            q.dv = p.dv;
+            q.dw = p.dw;
+        }
+    }
+
+    // write back the inner tile
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const Px &q = x[j];
+        const int lu = 2 * wave + j;
+        if (!q.on || lv < K || lv >= K + TVk || lu < K || lu >= K + TUk) continue;
+        const unsigned g = (unsigned)q.gv + (unsigned)q.gu * a.rows;
+        a.out[S_DU][g] = q.du; a.out[S_DV][g] = q.dv; a.out[S_DW][g] = q.dw;
+        a.out[S_DU_ACC][g] = q.ua; a.out[S_DV_ACC][g] = q.va; a.out[S_DW_ACC][g] = q.wa;
+        a.out[S_PD][g] = q.pd;
+#pragma unroll
+        for (int k = 0; k < 6; k++) a.out[S_PUU + k][g] = q.p[k];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_pd_fused(const PDArgs &a, hipStream_t s)
+{
+    if (a.rows == 0 || a.cols == 0) return hipSuccess;
+    const dim3 grid((a.rows + TV - 1) / TV, (a.cols + TU - 1) / TU);
+    hipLaunchKernelGGL(k_pd_fused, grid, dim3(TV * TU), 0, s, a);
+    return hipGetLastError();
+}
+
+bool pd_tb_supported(int k) { return k == 2 || k == 3 || k == 4 || k == 6 || k == 8; }
+
+template <int K>
+static hipError_t launch_tb(const PDArgs &a, hipStream_t s)
+{
+    constexpr int TVk = BV - 2 * K, TUk = BU - 2 * K;
+    const dim3 grid((a.rows + TVk - 1) / TVk, (a.cols + TUk - 1) / TUk);
+    hipLaunchKernelGGL(k_pd_tb<K>, grid, dim3(BT), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pd_tb(const PDArgs &a, int k, hipStream_t s)
+{
+    if (a.rows == 0 || a.cols == 0) return hipSuccess;
+    switch (k) {
+    case 2: return launch_tb<2>(a, s);
+    case 3: return launch_tb<3>(a, s);
+    case 4: return launch_tb<4>(a, s);
+    case 6: return launch_tb<6>(a, s);
+    case 8: return launch_tb<8>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace pdf

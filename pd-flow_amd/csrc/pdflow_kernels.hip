@@ -11,13 +11,12 @@
 #include <hip/hip_runtime.h>
 
 #include "pdflow_internal.hpp"
+#include "pdflow_pixel.hpp"
 
 namespace pdf {
 namespace {
 
 constexpr int kBlock = 256;
-
-__device__ __forceinline__ float rhyp(float a, float b) { return 1.f / sqrtf(a * a + b * b); }
 
 __device__ __forceinline__ unsigned pixel_index() { return blockIdx.x * kBlock + threadIdx.x; }
 
@@ -316,23 +315,6 @@ __global__ void __launch_bounds__(kBlock) k_warping(LevelArgs a)
 }
 
 // ------------------------------------------------------------------ K8 mu and steps
-__device__ __forceinline__ float sigma_pd_of(float mu_uv, float ddu, float ddv)
-{
-    return 1.f / (mu_uv * (1.f + fabsf(ddu) + fabsf(ddv)) + 1e-10f);   // .cu:791
-}
-__device__ __forceinline__ float sigma_p_of(float lam, float r)
-{
-    return 0.5f / (lam * r + 1e-10f);                                    // .cu:792-795
-}
-__device__ __forceinline__ float tau_uv_of(float mu_uv, float dd, float lam, float acu_r)
-{
-    return 1.f / (mu_uv * fabsf(dd) + lam * acu_r + 1e-10f);             // .cu:805-806
-}
-__device__ __forceinline__ float tau_w_of(float mu_uv, float lam, float acu_r)
-{
-    return 1.f / (mu_uv + lam * acu_r + 1e-10f);                         // .cu:807
-}
-
 __global__ void __launch_bounds__(kBlock) k_mu_steps(LevelArgs a)
 {
     // computeMu + computeStepSizes (.cu:781-808)
@@ -387,20 +369,6 @@ __global__ void __launch_bounds__(kBlock) k_gradient(LevelArgs a)
     }
 }
 
-__device__ __forceinline__ void project(float &a, float &b)
-{
-    // unit-disc projection (.cu:849-859)
-    const float m = rhyp(a, b);
-    if (m < 1.f) { a = a * m; b = b * m; }
-}
-
-__device__ __forceinline__ float clamp_pd(float pd)
-{
-    // .cu:837-846
-    if (fabsf(pd) > 1.f) return (pd > 1.f) ? 1.f : -1.f;
-    return pd;
-}
-
 __global__ void __launch_bounds__(kBlock) k_dual(LevelArgs a)
 {
     // .cu:813-887
@@ -428,14 +396,6 @@ __global__ void __launch_bounds__(kBlock) k_dual(LevelArgs a)
     a.f[PDF_PWU][index] = pwu; a.f[PDF_PWV][index] = pwv;
 }
 
-__device__ __forceinline__ float div_axis(float r0, float p0, float rm, float pm, bool first, bool last)
-{
-    // one axis of .cu:941-979
-    if (first) return r0 * p0;
-    if (last) return -rm * pm;
-    return r0 * p0 - rm * pm;
-}
-
 __global__ void __launch_bounds__(kBlock) k_divergence(LevelArgs a)
 {
     // .cu:935-980
@@ -458,36 +418,6 @@ __global__ void __launch_bounds__(kBlock) k_divergence(LevelArgs a)
     a.f[PDF_DIVPW][index] = dw;
 }
 
-struct Primal {
-    float du, dv, dw;
-};
-
-__device__ __forceinline__ Primal primal_update(float du, float dv, float dw, float tu, float tv,
-                                                float tw, float muv, float pd, float ddu,
-                                                float ddv, float divu, float divv, float divw,
-                                                float dct, float dcu, float dcv, float li,
-                                                float ld)
-{
-    // .cu:896-923
-    du += -tu * (muv * ddu * pd - li * divu);
-    dv += -tv * (muv * ddv * pd - li * divv);
-    dw += -tw * (-muv * pd - ld * divw);
-    const float of = dct + dcu * du + dcv * dv;
-    const float th = tu * dcu * dcu + tv * dcv * dcv;
-    if (of < -th) {
-        du += tu * dcu;
-        dv += tv * dcv;
-    } else if (of > th) {
-        du -= tu * dcu;
-        dv -= tv * dcv;
-    } else {
-        const float den = tu * dcu * dcu + tv * dcv * dcv + 1e-10f;
-        du -= tu * dcu * of / den;
-        dv -= tv * dcv * of / den;
-    }
-    return {du, dv, dw};
-}
-
 __global__ void __launch_bounds__(kBlock) k_primal(LevelArgs a)
 {
     // .cu:889-933
@@ -505,154 +435,6 @@ __global__ void __launch_bounds__(kBlock) k_primal(LevelArgs a)
     a.f[PDF_DU_NEW][index] = p.du;
     a.f[PDF_DV_NEW][index] = p.dv;
     a.f[PDF_DW_NEW][index] = p.dw;
-}
-
-// ------------------------------------------------------------------ fused K9-K12
-// One launch = one full primal-dual iteration on a 32x8 tile (256 threads, one pixel
-// each).  The dual step needs (acc+prev) at +1 in v and u; the primal step needs the new
-// duals at -1 in v and u.  The tile therefore stages (acc+prev) on a 1-pixel ring, and
-// recomputes the new TV duals of its low-side halo (one row and one column) instead of
-// synchronising with the neighbouring tiles.  Duals and acc are ping-ponged so halo reads
-// always see the previous iteration.  Step sizes are recomputed from ri/rj/mu/ddu/ddv,
-// bit-identical to the values K8 stores.
-constexpr int TV = 32, TU = 8;                 // tile rows (v) x cols (u)
-constexpr int RV = TV + 2, RU = TU + 2;        // ring region of (acc+prev)
-constexpr int HV = TV + 1, HU = TU + 1;        // tile + low halo (ri, rj, new duals)
-
-struct TVDual {
-    float puu, puv, pvu, pvv, pwu, pwv;
-};
-
-__global__ void __launch_bounds__(TV * TU) k_pd_fused(PDArgs a)
-{
-    __shared__ float sU[RU * RV], sV[RU * RV], sW[RU * RV];
-    __shared__ float sRi[HU * HV], sRj[HU * HV];
-    __shared__ float sP[6][HU * HV];
-
-    const int tid = threadIdx.x;
-    const int tv = tid % TV, tu = tid / TV;
-    const int v0 = blockIdx.x * TV, u0 = blockIdx.y * TU;
-    const int rows = (int)a.rows, cols = (int)a.cols;
-    const float li = a.lambda_i, ld = a.lambda_d;
-
-    // stage (acc + prev) on the ring region [v0-1, v0+TV] x [u0-1, u0+TU]
-    for (int e = tid; e < RU * RV; e += TV * TU) {
-        const int ev = e % RV - 1, eu = e / RV - 1;
-        const int gv = v0 + ev, gu = u0 + eu;
-        float su = 0.f, sv = 0.f, sw = 0.f;
-        if (gv >= 0 && gv < rows && gu >= 0 && gu < cols) {
-            const unsigned g = (unsigned)gv + (unsigned)gu * a.rows;
-            su = a.du_acc[g] + a.du_prev[g];
-            sv = a.dv_acc[g] + a.dv_prev[g];
-            sw = a.dw_acc[g];
-        }
-        sU[e] = su; sV[e] = sv; sW[e] = sw;
-    }
-    // stage ri, rj on tile + low halo [v0-1, v0+TV-1] x [u0-1, u0+TU-1]
-    for (int e = tid; e < HU * HV; e += TV * TU) {
-        const int ev = e % HV - 1, eu = e / HV - 1;
-        const int gv = v0 + ev, gu = u0 + eu;
-        float ri = 0.f, rj = 0.f;
-        if (gv >= 0 && gv < rows && gu >= 0 && gu < cols) {
-            const unsigned g = (unsigned)gv + (unsigned)gu * a.rows;
-            ri = a.ri[g];
-            rj = a.rj[g];
-        }
-        sRi[e] = ri; sRj[e] = rj;
-    }
-    __syncthreads();
-
-    // dual step of the TV duals at a pixel of the tile or its low halo (local ev, eu)
-    auto tv_dual = [&](int ev, int eu, unsigned g, int gv, int gu) -> TVDual {
-        const int r = (eu + 1) * RV + (ev + 1);          // ring index
-        const int h = (eu + 1) * HV + (ev + 1);          // halo-grid index
-        const float ri = sRi[h], rj = sRj[h];
-        const float u_c = sU[r], v_c = sV[r], w_c = sW[r];
-        float g1u = 0.f, g1v = 0.f, g1w = 0.f, g2u = 0.f, g2v = 0.f, g2w = 0.f;
-        if (gu != cols - 1) {
-            g1u = ri * (sU[r + RV] - u_c);
-            g1v = ri * (sV[r + RV] - v_c);
-            g1w = ri * (sW[r + RV] - w_c);
-        }
-        if (gv != rows - 1) {
-            g2u = rj * (sU[r + 1] - u_c);
-            g2v = rj * (sV[r + 1] - v_c);
-            g2w = rj * (sW[r + 1] - w_c);
-        }
-        const float sx = sigma_p_of(li, ri), sy = sigma_p_of(li, rj);
-        const float swx = 0.5f / (ri * ld + 1e-10f), swy = 0.5f / (rj * ld + 1e-10f);
-        TVDual p;
-        p.puu = a.puu[g] + sx * li * g1u;
-        p.puv = a.puv[g] + sy * li * g2u;
-        p.pvu = a.pvu[g] + sx * li * g1v;
-        p.pvv = a.pvv[g] + sy * li * g2v;
-        p.pwu = a.pwu[g] + swx * ld * g1w;
-        p.pwv = a.pwv[g] + swy * ld * g2w;
-        project(p.puu, p.puv);
-        project(p.pvu, p.pvv);
-        project(p.pwu, p.pwv);
-        return p;
-    };
-    auto store_dual = [&](int h, const TVDual &p) {
-        sP[0][h] = p.puu; sP[1][h] = p.puv; sP[2][h] = p.pvu;
-        sP[3][h] = p.pvv; sP[4][h] = p.pwu; sP[5][h] = p.pwv;
-    };
-
-    const int gv = v0 + tv, gu = u0 + tu;
-    const bool inside = gv < rows && gu < cols;
-    const unsigned g = (unsigned)gv + (unsigned)gu * a.rows;
-    const int h = (tu + 1) * HV + (tv + 1);
-    float pd = 0.f;
-    if (inside) {
-        const TVDual p = tv_dual(tv, tu, g, gv, gu);
-        store_dual(h, p);
-        a.puu_o[g] = p.puu; a.puv_o[g] = p.puv; a.pvu_o[g] = p.pvu;
-        a.pvv_o[g] = p.pvv; a.pwu_o[g] = p.pwu; a.pwv_o[g] = p.pwv;
-        const float mu_uv = a.mu_uv[g], ddu = a.ddu[g], ddv = a.ddv[g];
-        pd = a.pd[g] + sigma_pd_of(mu_uv, ddu, ddv) * mu_uv *
-                           (-a.dw_acc[g] + a.ddt[g] + ddu * a.du_acc[g] + ddv * a.dv_acc[g]);
-        pd = clamp_pd(pd);
-        a.pd[g] = pd;
-    }
-    // low halo: row v0-1 (TU pixels) and column u0-1 (TV pixels)
-    if (tid < TU + TV) {
-        int ev, eu;
-        if (tid < TU) { ev = -1; eu = tid; }
-        else { ev = tid - TU; eu = -1; }
-        const int hv = v0 + ev, hu = u0 + eu;
-        if (hv >= 0 && hu >= 0 && hv < rows && hu < cols) {
-            const unsigned hg = (unsigned)hv + (unsigned)hu * a.rows;
-            store_dual((eu + 1) * HV + (ev + 1), tv_dual(ev, eu, hg, hv, hu));
-        }
-    }
-    __syncthreads();
-
-    if (inside) {
-        const float ri = sRi[h], rj = sRj[h];
-        const float riu = sRi[h - HV], rjv = sRj[h - 1];   // (v, u-1) and (v-1, u)
-        const bool uf = gu == 0, ul = gu == cols - 1, vf = gv == 0, vl = gv == rows - 1;
-        float divu = div_axis(ri, sP[0][h], riu, sP[0][h - HV], uf, ul);
-        float divv = div_axis(ri, sP[2][h], riu, sP[2][h - HV], uf, ul);
-        float divw = div_axis(ri, sP[4][h], riu, sP[4][h - HV], uf, ul);
-        divu += div_axis(rj, sP[1][h], rjv, sP[1][h - 1], vf, vl);
-        divv += div_axis(rj, sP[3][h], rjv, sP[3][h - 1], vf, vl);
-        divw += div_axis(rj, sP[5][h], rjv, sP[5][h - 1], vf, vl);
-        float acu_r = ri + rj;
-        if (gu > 0) acu_r += riu;
-        if (gv > 0) acu_r += rjv;
-        const float mu_uv = a.mu_uv[g], ddu = a.ddu[g], ddv = a.ddv[g];
-        const float du0 = a.du[g], dv0 = a.dv[g], dw0 = a.dw[g];
-        const Primal p = primal_update(du0, dv0, dw0, tau_uv_of(mu_uv, ddu, li, acu_r),
-                                       tau_uv_of(mu_uv, ddv, li, acu_r), tau_w_of(mu_uv, ld, acu_r),
-                                       mu_uv, pd, ddu, ddv, divu, divv, divw, a.dct[g], a.dcu[g],
-                                       a.dcv[g], li, ld);
-        a.du_acc_o[g] = 2.f * p.du - du0;
-        a.dv_acc_o[g] = 2.f * p.dv - dv0;
-        a.dw_acc_o[g] = 2.f * p.dw - dw0;
-        a.du[g] = p.du;
-        a.dv[g] = p.dv;
-        a.dw[g] = p.dw;
-    }
 }
 
 // ------------------------------------------------------------------ K13/K14 filter
@@ -830,13 +612,5 @@ hipError_t launch_filter(const LevelArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 hipError_t launch_motion_field(const LevelArgs &a, hipStream_t s) { PDF_LAUNCH(k_motion_field, a.rows * a.cols, a, s); }
-
-hipError_t launch_pd_fused(const PDArgs &a, hipStream_t s)
-{
-    if (a.rows == 0 || a.cols == 0) return hipSuccess;
-    const dim3 grid((a.rows + TV - 1) / TV, (a.cols + TU - 1) / TU);
-    hipLaunchKernelGGL(k_pd_fused, grid, dim3(TV * TU), 0, s, a);
-    return hipGetLastError();
-}
 
 }  // namespace pdf

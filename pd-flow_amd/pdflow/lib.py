@@ -77,6 +77,7 @@ def hip_library():
         "pdf_allocate_level": ([vp, c_uint, c_uint, c_uint, c_uint], c_int),
         "pdf_free_level": ([vp], c_int),
         "pdf_pd_iterations": ([vp, c_uint], c_int),
+        "pdf_set_iteration_kernel": ([vp, c_int, c_uint], c_int),
         "pdf_copy_motion_field": ([vp, fp, fp, fp], c_int),
         "pdf_copy_all_solutions": ([vp] + [fp] * 11, c_int),
         "pdf_motion_field_device": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)], c_int),
@@ -220,6 +221,10 @@ class Solver:
 
     def pd_iterations(self, n):
         self._chk(self.lib.pdf_pd_iterations(self.ctx, n))
+
+    def set_iteration_kernel(self, k, min_pixels=0):
+        """k iterations per launch (temporal blocking) on levels of >= min_pixels pixels."""
+        self._chk(self.lib.pdf_set_iteration_kernel(self.ctx, k, min_pixels))
 
     def solve(self, iters=None, graph=False):
         arr = None if iters is None else (ctypes.c_uint * len(iters))(*iters)

@@ -203,3 +203,27 @@ def test_nan_and_empty_depth_inputs_do_not_break_the_solve(gpu_available):
     b = Oracle(120, 4, threads=8).solve_pair(I1, Z1, I2, Z2)
     for x, y, n in zip(a, b, ("dx", "dy", "dz")):
         assert_same(x, y, n)
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 6, 8])
+def test_temporal_blocking_matches_oracle(gpu_available, k):
+    """k iterations per launch on every level (min_pixels=0), including remainders, against
+    the oracle's one-stage-at-a-time iterations: bit-identical expected."""
+    say(f"temporal blocking k={k}")
+    g, o, _ = both(240, 2, seed=31 + k)
+    g.set_iteration_kernel(k, 0)
+    for (i, rows_i, cols_i, limg, iters) in level_table(240, 2):
+        for s in (g, o):
+            s.allocate_level(rows_i, cols_i, i, limg)
+            for st in ["assign_zeros"] + (["upsample"] if i > 0 else []) + SETUP_STAGES:
+                s.stage(st)
+        for n in (k + 1, iters - k - 1):
+            g.pd_iterations(n)
+            o.pd_iterations(n)
+            for f in STAGE_FIELDS["iterate"]:
+                assert_same(g.get_field(f), o.get_field(f), f"k={k} level {i} {f}")
+        for s in (g, o):
+            s.stage("filter")
+            s.stage("motion_field")
+    for a, b, n in zip(g.motion_field(), o.motion_field(), ("dx", "dy", "dz")):
+        assert_same(a, b, n)
