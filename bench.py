@@ -57,6 +57,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--tb-k", type=int, default=None,
+                    help="iterations per launch of the temporally blocked kernel (1 = off)")
+    ap.add_argument("--tb-min", type=int, default=None, help="smallest level (pixels) it runs on")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -87,6 +90,9 @@ def main():
         bufs.append(b)
     s = Solver(rows, cam_mode)
     lib, ctx = s.lib, s.ctx
+    if args.tb_k is not None or args.tb_min is not None:
+        s.set_iteration_kernel(args.tb_k if args.tb_k is not None else 4,
+                               args.tb_min if args.tb_min is not None else 120 * 160)
     arr = (np.ctypeslib.ctypes.c_uint * len(iters))(*iters)
     use_graph = 0 if args.no_graph else 1
 

@@ -98,8 +98,15 @@ int pdf_pd_iterations(pdf_ctx *ctx, unsigned n);
 /* Iteration-kernel selection (no reference counterpart; the reference launches four kernels
  * per iteration): levels of at least min_pixels pixels run k iterations per launch in the
  * temporally blocked kernel (k in {2,3,4,6,8}); k <= 1 disables it.  Results are identical
- * for every setting.  Default: k = 4 from 120x160 up. */
+ * for every setting.  Default: k = 4 from 60x80 up. */
 int pdf_set_iteration_kernel(pdf_ctx *ctx, int k, unsigned min_pixels);
+
+/* Solve-path options (no reference counterpart).  fused_setup: pdf_solve runs each level's
+ * set-up in four fused launches (default) instead of the reference's eleven bridges; the
+ * fused path leaves the sigma_*, tau_*, *_upsamp, grad* and divp* planes unwritten.
+ * level_kernel: levels of at most 2048 pixels run all their iterations in one launch
+ * (default on).  Results are identical for every setting. */
+int pdf_set_solve_options(pdf_ctx *ctx, int fused_setup, int level_kernel);
 
 int pdf_filter(pdf_ctx *ctx);              /* FilterBridge         .cu:1237-1241 */
 int pdf_motion_field(pdf_ctx *ctx);        /* MotionFieldBridge    .cu:1243-1246 */

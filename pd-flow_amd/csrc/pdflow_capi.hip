@@ -77,7 +77,11 @@ struct pdf_ctx {
     // iteration kernel selection: levels of at least tb_min_pixels pixels run tb_k
     // iterations per launch (temporal blocking); 0 disables it
     int tb_k = 4;
-    unsigned tb_min_pixels = 120 * 160;
+    unsigned tb_min_pixels = 60 * 80;
+    // levels of at most 2048 pixels run all their iterations in one workgroup
+    bool level_kernel = true;
+    // pdf_solve uses the fused set-up kernels (false: the reference's stage-by-stage bridges)
+    bool fused_setup = true;
 
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -457,10 +461,32 @@ static int pd_launch(pdf_ctx *c, int k)
     return PDF_OK;
 }
 
+// all n iterations in one single-workgroup launch (levels of at most 2048 pixels)
+static int pd_launch_level(pdf_ctx *c, unsigned n)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        HIP_TRY(c, hipEventCreate(&e0));
+        HIP_TRY(c, hipEventCreate(&e1));
+        HIP_TRY(c, hipEventRecord(e0, c->stream));
+    }
+    int rc = check_launch(c, pdf::launch_pd_level(pd_args(c), n, c->stream), "whole-level primal-dual");
+    if (rc) return rc;
+    if (c->timing) {
+        HIP_TRY(c, hipEventRecord(e1, c->stream));
+        c->events.emplace_back(e0, e1);
+        c->event_pix.push_back((uint64_t)c->rows_i * c->cols_i * (uint64_t)n);
+    }
+    swap_pingpong(c);
+    return PDF_OK;
+}
+
 int pdf_pd_iterations(pdf_ctx *c, unsigned n)
 {
     int rc = need_level(c);
     if (rc) return rc;
+    if (n == 0) return PDF_OK;
+    if (c->level_kernel && pdf::pd_level_supported(c->rows_i * c->cols_i)) return pd_launch_level(c, n);
     const bool tb = c->tb_k > 1 && (size_t)c->rows_i * c->cols_i >= c->tb_min_pixels;
     unsigned done = 0;
     if (tb)
@@ -526,6 +552,18 @@ int pdf_motion_field_device(pdf_ctx *c, const float **dx, const float **dy, cons
     return PDF_OK;
 }
 
+static int fused_level(pdf_ctx *c, unsigned i, const unsigned *iters)
+{
+    // the same stages as the bridges below, in four set-up launches (pdflow_setup.hip)
+    const pdf::LevelArgs a = level_args(c);
+    int rc = check_launch(c, pdf::launch_level_init(a, i > 0, c->stream), "level init");
+    if (!rc) rc = check_launch(c, pdf::launch_rij_gradients(a, c->stream), "rij + gradients");
+    if (!rc) rc = check_launch(c, pdf::launch_warp_mu(a, false, c->stream), "warp + mu");
+    if (!rc) rc = pdf_pd_iterations(c, iters[i]);
+    if (!rc) rc = check_launch(c, pdf::launch_filter_motion(level_args(c), c->stream), "filter + motion");
+    return rc;
+}
+
 static int solve_sequence(pdf_ctx *c, const unsigned *iters)
 {
     // scene_flow_impair.cpp:103-167, one level at a time, coarse to fine
@@ -533,15 +571,20 @@ static int solve_sequence(pdf_ctx *c, const unsigned *iters)
         unsigned rows_i, cols_i, level_image;
         level_dims_for(c, i, &rows_i, &cols_i, &level_image);
         int rc = pdf_allocate_level(c, rows_i, cols_i, i, level_image);
-        if (!rc) rc = pdf_assign_zeros(c);
-        if (!rc && i > 0) rc = pdf_upsample(c);
-        if (!rc) rc = pdf_rij(c);
-        if (!rc) rc = pdf_image_gradients(c);
-        if (!rc) rc = pdf_warping(c);
-        if (!rc) rc = pdf_mu_and_step_sizes(c);
-        if (!rc) rc = pdf_pd_iterations(c, iters[i]);
-        if (!rc) rc = pdf_filter(c);
-        if (!rc) rc = pdf_motion_field(c);
+        if (rc) return rc;
+        if (c->fused_setup) {
+            rc = fused_level(c, i, iters);
+        } else {
+            rc = pdf_assign_zeros(c);
+            if (!rc && i > 0) rc = pdf_upsample(c);
+            if (!rc) rc = pdf_rij(c);
+            if (!rc) rc = pdf_image_gradients(c);
+            if (!rc) rc = pdf_warping(c);
+            if (!rc) rc = pdf_mu_and_step_sizes(c);
+            if (!rc) rc = pdf_pd_iterations(c, iters[i]);
+            if (!rc) rc = pdf_filter(c);
+            if (!rc) rc = pdf_motion_field(c);
+        }
         if (rc) return rc;
     }
     return PDF_OK;
@@ -602,16 +645,54 @@ int pdf_solve_graph(pdf_ctx *c, const unsigned *num_max_iter)
     return PDF_OK;
 }
 
+static pdf::PyrArgs pyr_level_args(const pdf_ctx *c, unsigned i, bool old_frame, const float *I0,
+                                   const float *Z0)
+{
+    const int ck = old_frame ? PDF_PYR_COLOUR_OLD : PDF_PYR_COLOUR;
+    const int dk = old_frame ? PDF_PYR_DEPTH_OLD : PDF_PYR_DEPTH;
+    pdf::PyrArgs a;
+    // .cu:1176-1177
+    a.cols_i = (unsigned)(640.f / ((float)c->cam_mode * powf(2.f, (float)i)));
+    a.rows_i = (unsigned)(480.f / ((float)c->cam_mode * powf(2.f, (float)i)));
+    a.level = i;
+    a.inv_f_i = 2.f * c->tan_half / (float)a.cols_i;
+    a.colp = i ? c->pyr[ck][i - 1] : nullptr;
+    a.depp = i ? c->pyr[dk][i - 1] : nullptr;
+    a.colour_wf = I0;
+    a.depth_wf = Z0;
+    a.col = c->pyr[ck][i];
+    a.dep = c->pyr[dk][i];
+    a.xx = c->pyr[old_frame ? PDF_PYR_XX_OLD : PDF_PYR_XX][i];
+    a.yy = c->pyr[old_frame ? PDF_PYR_YY_OLD : PDF_PYR_YY][i];
+    std::memcpy(a.g, c->g_mask, sizeof(a.g));
+    return a;
+}
+
 int pdf_solve_pair_device(pdf_ctx *c, const float *dI1, const float *dZ1, const float *dI2,
                           const float *dZ2, const unsigned *num_max_iter, int use_graph)
 {
+    if (!c || !dI1 || !dZ1 || !dI2 || !dZ2) return fail(c, PDF_ERR_ARG, "null frame pointer");
+    // Equivalent to copyNewFrames(frame 1) + pyramid + copyNewFrames(frame 2) + pyramid: two
+    // swaps leave the old/new slot assignment unchanged, frame 1's pyramid lands in the "old"
+    // planes and frame 2's in the "new" ones; level 0 reads the caller's planes directly.
+    for (unsigned i = 0; i < c->pyr_levels; i++) {
+        pdf::Pyr2Args a;
+        a.frame[0] = pyr_level_args(c, i, true, dI1, dZ1);
+        a.frame[1] = pyr_level_args(c, i, false, dI2, dZ2);
+        int rc = check_launch(c, pdf::launch_pyramid2(a, c->stream), "pair pyramid");
+        if (rc) return rc;
+    }
+    return use_graph ? pdf_solve_graph(c, num_max_iter) : pdf_solve(c, num_max_iter);
+}
+
+int pdf_set_solve_options(pdf_ctx *c, int fused_setup, int level_kernel)
+{
     if (!c) return PDF_ERR_ARG;
-    int rc = pdf_copy_new_frames_device(c, dI1, dZ1);
-    if (!rc) rc = pdf_gaussian_pyramid(c, c->pyr_levels, c->cam_mode);
-    if (!rc) rc = pdf_copy_new_frames_device(c, dI2, dZ2);
-    if (!rc) rc = pdf_gaussian_pyramid(c, c->pyr_levels, c->cam_mode);
-    if (!rc) rc = use_graph ? pdf_solve_graph(c, num_max_iter) : pdf_solve(c, num_max_iter);
-    return rc;
+    c->fused_setup = fused_setup != 0;
+    c->level_kernel = level_kernel != 0;
+    for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
+    c->graphs.clear();
+    return PDF_OK;
 }
 
 int pdf_get_field(pdf_ctx *c, int field, float *host_out)

@@ -28,6 +28,11 @@ struct PyrArgs {
     float g[25];
 };
 
+// Both frames of a pair: frame[0] builds the "old" pyramid, frame[1] the "new" one.
+struct Pyr2Args {
+    PyrArgs frame[2];
+};
+
 // State planes of the primal-dual iteration, in the order of PDArgs::in/out.
 enum PDState {
     S_DU = 0, S_DV, S_DW, S_DU_ACC, S_DV_ACC, S_DW_ACC, S_PD,
@@ -63,7 +68,16 @@ hipError_t launch_primal(const LevelArgs &a, hipStream_t s);
 hipError_t launch_pd_fused(const PDArgs &a, hipStream_t s);
 // k fused iterations in one launch (temporal blocking); k must satisfy pd_tb_supported
 hipError_t launch_pd_tb(const PDArgs &a, int k, hipStream_t s);
+// all n iterations of a level of at most 2048 pixels in one single-workgroup launch
+hipError_t launch_pd_level(const PDArgs &a, unsigned n, hipStream_t s);
+bool pd_level_supported(unsigned pixels);
 hipError_t launch_filter(const LevelArgs &a, hipStream_t s);
+// fused set-up of the solve path (pdflow_setup.hip)
+hipError_t launch_level_init(const LevelArgs &a, bool has_coarse, hipStream_t s);
+hipError_t launch_rij_gradients(const LevelArgs &a, hipStream_t s);
+hipError_t launch_warp_mu(const LevelArgs &a, bool write_steps, hipStream_t s);
+hipError_t launch_filter_motion(const LevelArgs &a, hipStream_t s);
+hipError_t launch_pyramid2(const Pyr2Args &a, hipStream_t s);
 hipError_t launch_motion_field(const LevelArgs &a, hipStream_t s);
 
 }  // namespace pdf

@@ -179,10 +179,9 @@ __global__ void __launch_bounds__(TV * TU) k_pd_fused(PDArgs a)
     }
 }
 
-// ------------------------------------------------------------------ K iterations per launch
-constexpr int BV = 64, BU = 32;        // region: one wave64 per column pair, 16 waves
-constexpr int BT = 1024;               // threads per workgroup, two pixels (columns) each
-
+// ------------------------------------------------------------------ register-resident pixels
+// A pixel whose whole iteration state lives in registers across iterations (the
+// temporally blocked and whole-level kernels).  Neighbour values travel through LDS.
 struct Px {
     // state
     float du, dv, dw, ua, va, wa, pd, p[6];
@@ -193,6 +192,114 @@ struct Px {
     int gv, gu;
     bool on;
 };
+
+__device__ __forceinline__ void load_px(Px &q, const PDArgs &a, int gv, int gu, bool on)
+{
+    // loads are unconditional (pixel 0 stands in for pixels outside the image) and then
+    // selected, so every value stays in a register
+    q.gv = gv;
+    q.gu = gu;
+    q.on = on;
+    const unsigned g = on ? (unsigned)gv + (unsigned)gu * a.rows : 0u;
+    auto ld = [&](const float *plane, float off) { return on ? plane[g] : off; };
+    q.du = ld(a.in[S_DU], 0.f); q.dv = ld(a.in[S_DV], 0.f); q.dw = ld(a.in[S_DW], 0.f);
+    q.ua = ld(a.in[S_DU_ACC], 0.f); q.va = ld(a.in[S_DV_ACC], 0.f); q.wa = ld(a.in[S_DW_ACC], 0.f);
+    q.pd = ld(a.in[S_PD], 0.f);
+#pragma unroll
+    for (int k = 0; k < 6; k++) q.p[k] = ld(a.in[S_PUU + k], 0.f);
+    q.up = ld(a.du_prev, 0.f); q.vp = ld(a.dv_prev, 0.f); q.mu = ld(a.mu_uv, 0.f);
+    q.ddu = ld(a.ddu, 0.f); q.ddv = ld(a.ddv, 0.f); q.ddt = ld(a.ddt, 0.f);
+    q.dct = ld(a.dct, 0.f); q.dcu = ld(a.dcu, 0.f); q.dcv = ld(a.dcv, 0.f);
+    q.ri = ld(a.ri, 1.f); q.rj = ld(a.rj, 1.f);
+}
+
+__device__ __forceinline__ void store_px(const Px &q, const PDArgs &a)
+{
+    const unsigned g = (unsigned)q.gv + (unsigned)q.gu * a.rows;
+    a.out[S_DU][g] = q.du; a.out[S_DV][g] = q.dv; a.out[S_DW][g] = q.dw;
+    a.out[S_DU_ACC][g] = q.ua; a.out[S_DV_ACC][g] = q.va; a.out[S_DW_ACC][g] = q.wa;
+    a.out[S_PD][g] = q.pd;
+#pragma unroll
+    for (int k = 0; k < 6; k++) a.out[S_PUU + k][g] = q.p[k];
+}
+
+// riu = ri at (v, u-1), rjv = rj at (v-1, u); only read when those pixels exist
+__device__ __forceinline__ void derive_px(Px &q, float riu, float rjv, float li, float ld)
+{
+    // the first products of the reference expressions (.cu:822-834) are per-launch constants
+    q.sxl = sigma_p_of(li, q.ri) * li;
+    q.syl = sigma_p_of(li, q.rj) * li;
+    q.swxl = (0.5f / (q.ri * ld + 1e-10f)) * ld;
+    q.swyl = (0.5f / (q.rj * ld + 1e-10f)) * ld;
+    q.spdm = sigma_pd_of(q.mu, q.ddu, q.ddv) * q.mu;
+    float acu_r = q.ri + q.rj;                                   // .cu:801-803
+    if (q.gu > 0) acu_r += riu;
+    if (q.gv > 0) acu_r += rjv;
+    q.tu = tau_uv_of(q.mu, q.ddu, li, acu_r);
+    q.tv = tau_uv_of(q.mu, q.ddv, li, acu_r);
+    q.tw = tau_w_of(q.mu, ld, acu_r);
+}
+
+// dual step (.cu:813-887 with the gradient of .cu:982-1013): n* are (acc+prev) at (v, u+1),
+// m* at (v+1, u); read only when those pixels exist
+__device__ __forceinline__ void dual_px(Px &q, int rows, int cols, float nU, float nV, float nW,
+                                        float mU, float mV, float mW)
+{
+    const float u_c = q.ua + q.up, v_c = q.va + q.vp, w_c = q.wa;
+    float g1u = 0.f, g1v = 0.f, g1w = 0.f, g2u = 0.f, g2v = 0.f, g2w = 0.f;
+    if (q.gu != cols - 1) {
+        g1u = q.ri * (nU - u_c);
+        g1v = q.ri * (nV - v_c);
+        g1w = q.ri * (nW - w_c);
+    }
+    if (q.gv != rows - 1) {
+        g2u = q.rj * (mU - u_c);
+        g2v = q.rj * (mV - v_c);
+        g2w = q.rj * (mW - w_c);
+    }
+    q.p[0] = q.p[0] + q.sxl * g1u;
+    q.p[1] = q.p[1] + q.syl * g2u;
+    q.p[2] = q.p[2] + q.sxl * g1v;
+    q.p[3] = q.p[3] + q.syl * g2v;
+    q.p[4] = q.p[4] + q.swxl * g1w;
+    q.p[5] = q.p[5] + q.swyl * g2w;
+    project(q.p[0], q.p[1]);
+    project(q.p[2], q.p[3]);
+    project(q.p[4], q.p[5]);
+    q.pd = clamp_pd(q.pd + q.spdm * (-q.wa + q.ddt + q.ddu * q.ua + q.ddv * q.va));
+}
+
+// primal step (.cu:889-933 with the divergence of .cu:935-980): pu* are the new duals
+// puu, pvu, pwu at (v, u-1) with riu = ri there; pv* are puv, pvv, pwv at (v-1, u) with
+// rjv = rj there; read only when those pixels exist
+__device__ __forceinline__ void primal_px(Px &q, int rows, int cols, float riu, float puu,
+                                          float pvu, float pwu, float rjv, float puv, float pvv,
+                                          float pwv, float li, float ld)
+{
+    const bool uf = q.gu == 0, ul = q.gu == cols - 1, vf = q.gv == 0, vl = q.gv == rows - 1;
+    float divu = div_axis(q.ri, q.p[0], riu, puu, uf, ul);
+    float divv = div_axis(q.ri, q.p[2], riu, pvu, uf, ul);
+    float divw = div_axis(q.ri, q.p[4], riu, pwu, uf, ul);
+    divu += div_axis(q.rj, q.p[1], rjv, puv, vf, vl);
+    divv += div_axis(q.rj, q.p[3], rjv, pvv, vf, vl);
+    divw += div_axis(q.rj, q.p[5], rjv, pwv, vf, vl);
+    const float du0 = q.du, dv0 = q.dv, dw0 = q.dw;
+    const Primal p = primal_update(du0, dv0, dw0, q.tu, q.tv, q.tw, q.mu, q.pd, q.ddu, q.ddv, divu,
+                                   divv, divw, q.dct, q.dcu, q.dcv, li, ld);
+    q.ua = 2.f * p.du - du0;
+    q.va = 2.f * p.dv - dv0;
+    q.wa = 2.f * p.dw - dw0;
+    q.du = p.du;
+    q.dv = p.dv;
+    q.dw = p.dw;
+}
+
+// ------------------------------------------------------------------ K iterations per launch
+constexpr int BV = 64, BU = 32;        // region: one wave64 per column pair, 16 waves
+constexpr int BT = 1024;               // threads per workgroup, two pixels (columns) each
+
+// applies f(pixel, j) to a thread's two pixels (named variables, so they stay in registers)
+#define PD_BOTH(f) do { f(x0, 0); f(x1, 1); } while (0)
 
 template <int K>
 __global__ void __launch_bounds__(BT) k_pd_tb(PDArgs a)
@@ -208,137 +315,135 @@ __global__ void __launch_bounds__(BT) k_pd_tb(PDArgs a)
     const float li = a.lambda_i, ld = a.lambda_d;
     const int lv = lane;
 
-    Px x[2];
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-        Px &q = x[j];
+    Px x0, x1;
+    auto load = [&](Px &q, int j) __attribute__((always_inline)) {
         const int lu = 2 * wave + j;
-        q.gv = rv0 + lv;
-        q.gu = ru0 + lu;
-        q.on = q.gv >= 0 && q.gv < rows && q.gu >= 0 && q.gu < cols;
-        const unsigned g = q.on ? (unsigned)q.gv + (unsigned)q.gu * a.rows : 0u;
-        if (q.on) {
-            q.du = a.in[S_DU][g]; q.dv = a.in[S_DV][g]; q.dw = a.in[S_DW][g];
-            q.ua = a.in[S_DU_ACC][g]; q.va = a.in[S_DV_ACC][g]; q.wa = a.in[S_DW_ACC][g];
-            q.pd = a.in[S_PD][g];
-#pragma unroll
-            for (int k = 0; k < 6; k++) q.p[k] = a.in[S_PUU + k][g];
-            q.up = a.du_prev[g]; q.vp = a.dv_prev[g]; q.mu = a.mu_uv[g];
-            q.ddu = a.ddu[g]; q.ddv = a.ddv[g]; q.ddt = a.ddt[g];
-            q.dct = a.dct[g]; q.dcu = a.dcu[g]; q.dcv = a.dcv[g];
-            q.ri = a.ri[g]; q.rj = a.rj[g];
-        } else {
-            q.du = q.dv = q.dw = q.ua = q.va = q.wa = q.pd = 0.f;
-#pragma unroll
-            for (int k = 0; k < 6; k++) q.p[k] = 0.f;
-            q.up = q.vp = q.mu = q.ddu = q.ddv = q.ddt = q.dct = q.dcu = q.dcv = 0.f;
-            q.ri = q.rj = 1.f;
-        }
+        const int gv = rv0 + lv, gu = ru0 + lu;
+        load_px(q, a, gv, gu, gv >= 0 && gv < rows && gu >= 0 && gu < cols);
         sRi[lu * BV + lv] = q.ri;
         sRj[lu * BV + lv] = q.rj;
-    }
+    };
+    PD_BOTH(load);
     __syncthreads();
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-        Px &q = x[j];
-        const int idx = (2 * wave + j) * BV + lv;
-        // first products of the reference expressions (.cu:822-834) are constant per launch
-        q.sxl = sigma_p_of(li, q.ri) * li;
-        q.syl = sigma_p_of(li, q.rj) * li;
-        q.swxl = (0.5f / (q.ri * ld + 1e-10f)) * ld;
-        q.swyl = (0.5f / (q.rj * ld + 1e-10f)) * ld;
-        q.spdm = sigma_pd_of(q.mu, q.ddu, q.ddv) * q.mu;
-        float acu_r = q.ri + q.rj;
-        if (q.gu > 0) acu_r += sRi[idx >= BV ? idx - BV : idx];
-        if (q.gv > 0) acu_r += sRj[lv > 0 ? idx - 1 : idx];
-        q.tu = tau_uv_of(q.mu, q.ddu, li, acu_r);
-        q.tv = tau_uv_of(q.mu, q.ddv, li, acu_r);
-        q.tw = tau_w_of(q.mu, ld, acu_r);
-    }
+    // neighbours outside the region only feed pixels of the discarded band, so their
+    // indices are clamped into the region
+    auto derive = [&](Px &q, int j) __attribute__((always_inline)) {
+        const int lu = 2 * wave + j, idx = lu * BV + lv;
+        derive_px(q, sRi[lu > 0 ? idx - BV : idx], sRj[lv > 0 ? idx - 1 : idx], li, ld);
+    };
+    PD_BOTH(derive);
 
+    auto put_bar = [&](Px &q, int j) __attribute__((always_inline)) {
+        const int idx = (2 * wave + j) * BV + lv;
+        sU[idx] = q.ua + q.up;
+        sV[idx] = q.va + q.vp;
+        sW[idx] = q.wa;
+    };
+    // pixels outside the image skip the arithmetic: no pixel inside reads them
+    auto dual = [&](Px &q, int j) __attribute__((always_inline)) {
+        if (!q.on) return;
+        const int lu = 2 * wave + j, idx = lu * BV + lv;
+        const int iu = lu + 1 < BU ? idx + BV : idx;
+        const int iv = lv + 1 < BV ? idx + 1 : idx;
+        dual_px(q, rows, cols, sU[iu], sV[iu], sW[iu], sU[iv], sV[iv], sW[iv]);
+#pragma unroll
+        for (int k = 0; k < 6; k++) sP[k][idx] = q.p[k];
+    };
+    auto primal = [&](Px &q, int j) __attribute__((always_inline)) {
+        if (!q.on) return;
+        const int lu = 2 * wave + j, idx = lu * BV + lv;
+        const int ju = lu > 0 ? idx - BV : idx;
+        const int jv = lv > 0 ? idx - 1 : idx;
+        primal_px(q, rows, cols, sRi[ju], sP[0][ju], sP[2][ju], sP[4][ju], sRj[jv], sP[1][jv],
+                  sP[3][jv], sP[5][jv], li, ld);
+    };
     for (int it = 0; it < K; it++) {
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int idx = (2 * wave + j) * BV + lv;
-            sU[idx] = x[j].ua + x[j].up;
-            sV[idx] = x[j].va + x[j].vp;
-            sW[idx] = x[j].wa;
-        }
+        PD_BOTH(put_bar);
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            Px &q = x[j];
-            const int lu = 2 * wave + j;
-            const int idx = lu * BV + lv;
-            const float u_c = sU[idx], v_c = sV[idx], w_c = sW[idx];
-            // neighbours outside the region only feed pixels of the discarded band
-            const int iu = lu + 1 < BU ? idx + BV : idx;
-            const int iv = lv + 1 < BV ? idx + 1 : idx;
-            float g1u = 0.f, g1v = 0.f, g1w = 0.f, g2u = 0.f, g2v = 0.f, g2w = 0.f;
-            if (q.gu != cols - 1) {
-                g1u = q.ri * (sU[iu] - u_c);
-                g1v = q.ri * (sV[iu] - v_c);
-                g1w = q.ri * (sW[iu] - w_c);
-            }
-            if (q.gv != rows - 1) {
-                g2u = q.rj * (sU[iv] - u_c);
-                g2v = q.rj * (sV[iv] - v_c);
-                g2w = q.rj * (sW[iv] - w_c);
-            }
-            q.p[0] = q.p[0] + q.sxl * g1u;
-            q.p[1] = q.p[1] + q.syl * g2u;
-            q.p[2] = q.p[2] + q.sxl * g1v;
-            q.p[3] = q.p[3] + q.syl * g2v;
-            q.p[4] = q.p[4] + q.swxl * g1w;
-            q.p[5] = q.p[5] + q.swyl * g2w;
-            project(q.p[0], q.p[1]);
-            project(q.p[2], q.p[3]);
-            project(q.p[4], q.p[5]);
-            q.pd = clamp_pd(q.pd + q.spdm * (-q.wa + q.ddt + q.ddu * q.ua + q.ddv * q.va));
-#pragma unroll
-            for (int k = 0; k < 6; k++) sP[k][idx] = q.p[k];
-        }
+        PD_BOTH(dual);
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            Px &q = x[j];
-            const int lu = 2 * wave + j;
-            const int idx = lu * BV + lv;
-            const int ju = lu > 0 ? idx - BV : idx;
-            const int jv = lv > 0 ? idx - 1 : idx;
-            const float riu = sRi[ju], rjv = sRj[jv];
-            const bool uf = q.gu == 0, ul = q.gu == cols - 1, vf = q.gv == 0, vl = q.gv == rows - 1;
-            float divu = div_axis(q.ri, q.p[0], riu, sP[0][ju], uf, ul);
-            float divv = div_axis(q.ri, q.p[2], riu, sP[2][ju], uf, ul);
-            float divw = div_axis(q.ri, q.p[4], riu, sP[4][ju], uf, ul);
-            divu += div_axis(q.rj, q.p[1], rjv, sP[1][jv], vf, vl);
-            divv += div_axis(q.rj, q.p[3], rjv, sP[3][jv], vf, vl);
-            divw += div_axis(q.rj, q.p[5], rjv, sP[5][jv], vf, vl);
-            const float du0 = q.du, dv0 = q.dv, dw0 = q.dw;
-            const Primal p = primal_update(du0, dv0, dw0, q.tu, q.tv, q.tw, q.mu, q.pd, q.ddu, q.ddv,
-                                           divu, divv, divw, q.dct, q.dcu, q.dcv, li, ld);
-            q.ua = 2.f * p.du - du0;
-            q.va = 2.f * p.dv - dv0;
-            q.wa = 2.f * p.dw - dw0;
-            q.du = p.du;
-            q.dv = p.dv;
-            q.dw = p.dw;
-        }
+        PD_BOTH(primal);
     }
 
     // write back the inner tile
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-        const Px &q = x[j];
+    auto store = [&](Px &q, int j) __attribute__((always_inline)) {
         const int lu = 2 * wave + j;
-        if (!q.on || lv < K || lv >= K + TVk || lu < K || lu >= K + TUk) continue;
-        const unsigned g = (unsigned)q.gv + (unsigned)q.gu * a.rows;
-        a.out[S_DU][g] = q.du; a.out[S_DV][g] = q.dv; a.out[S_DW][g] = q.dw;
-        a.out[S_DU_ACC][g] = q.ua; a.out[S_DV_ACC][g] = q.va; a.out[S_DW_ACC][g] = q.wa;
-        a.out[S_PD][g] = q.pd;
+        if (q.on && lv >= K && lv < K + TVk && lu >= K && lu < K + TUk) store_px(q, a);
+    };
+    PD_BOTH(store);
+}
+
+// ------------------------------------------------------------------ whole level per launch
+// Levels of at most 2048 pixels (15x20, 30x40) fit one 1024-thread workgroup with two pixels
+// per thread in column-major order: all n iterations run in one launch, with no halo since
+// the image borders are the only borders.
+constexpr int kLevelMaxPixels = 2 * BT;
+
+__global__ void __launch_bounds__(BT) k_pd_level(PDArgs a, int n)
+{
+    __shared__ float sU[kLevelMaxPixels], sV[kLevelMaxPixels], sW[kLevelMaxPixels];
+    __shared__ float sP[6][kLevelMaxPixels];
+    __shared__ float sRi[kLevelMaxPixels], sRj[kLevelMaxPixels];
+    const int rows = (int)a.rows, cols = (int)a.cols, P = rows * cols;
+    const float li = a.lambda_i, ld = a.lambda_d;
+
+    Px x0, x1;
+    auto load = [&](Px &q, int j) __attribute__((always_inline)) {
+        const int p = (int)threadIdx.x + j * BT;
+        const bool on = p < P;
+        load_px(q, a, on ? p % rows : 0, on ? p / rows : 0, on);
+        if (on) {
+            sRi[p] = q.ri;
+            sRj[p] = q.rj;
+        }
+    };
+    PD_BOTH(load);
+    __syncthreads();
+    // a pixel reads (v, u-1) only if u > 0 and (v-1, u) only if v > 0: indices stay in [0, P)
+    auto derive = [&](Px &q, int j) __attribute__((always_inline)) {
+        const int p = (int)threadIdx.x + j * BT;
+        if (q.on) derive_px(q, sRi[q.gu > 0 ? p - rows : p], sRj[q.gv > 0 ? p - 1 : p], li, ld);
+    };
+    PD_BOTH(derive);
+    auto put_bar = [&](Px &q, int j) __attribute__((always_inline)) {
+        const int p = (int)threadIdx.x + j * BT;
+        if (q.on) {
+            sU[p] = q.ua + q.up;
+            sV[p] = q.va + q.vp;
+            sW[p] = q.wa;
+        }
+    };
+    auto dual = [&](Px &q, int j) __attribute__((always_inline)) {
+        const int p = (int)threadIdx.x + j * BT;
+        if (!q.on) return;
+        const int iu = q.gu != cols - 1 ? p + rows : p;
+        const int iv = q.gv != rows - 1 ? p + 1 : p;
+        dual_px(q, rows, cols, sU[iu], sV[iu], sW[iu], sU[iv], sV[iv], sW[iv]);
 #pragma unroll
-        for (int k = 0; k < 6; k++) a.out[S_PUU + k][g] = q.p[k];
+        for (int k = 0; k < 6; k++) sP[k][p] = q.p[k];
+    };
+    auto primal = [&](Px &q, int j) __attribute__((always_inline)) {
+        const int p = (int)threadIdx.x + j * BT;
+        if (!q.on) return;
+        const int ju = q.gu > 0 ? p - rows : p;
+        const int jv = q.gv > 0 ? p - 1 : p;
+        primal_px(q, rows, cols, sRi[ju], sP[0][ju], sP[2][ju], sP[4][ju], sRj[jv], sP[1][jv],
+                  sP[3][jv], sP[5][jv], li, ld);
+    };
+    // per iteration: the barrier after put_bar orders the previous primal reads of sP before
+    // this dual's writes; the barrier after dual orders this dual's reads of sU/sV/sW before
+    // the next put_bar
+    for (int it = 0; it < n; it++) {
+        PD_BOTH(put_bar);
+        __syncthreads();
+        PD_BOTH(dual);
+        __syncthreads();
+        PD_BOTH(primal);
     }
+    auto store = [&](Px &q, int) __attribute__((always_inline)) {
+        if (q.on) store_px(q, a);
+    };
+    PD_BOTH(store);
 }
 
 }  // namespace
@@ -350,6 +455,16 @@ hipError_t launch_pd_fused(const PDArgs &a, hipStream_t s)
     hipLaunchKernelGGL(k_pd_fused, grid, dim3(TV * TU), 0, s, a);
     return hipGetLastError();
 }
+
+hipError_t launch_pd_level(const PDArgs &a, unsigned n, hipStream_t s)
+{
+    if (a.rows * a.cols > (unsigned)kLevelMaxPixels) return hipErrorInvalidValue;
+    if (a.rows == 0 || a.cols == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pd_level, dim3(1), dim3(BT), 0, s, a, (int)n);
+    return hipGetLastError();
+}
+
+bool pd_level_supported(unsigned pixels) { return pixels <= (unsigned)kLevelMaxPixels; }
 
 bool pd_tb_supported(int k) { return k == 2 || k == 3 || k == 4 || k == 6 || k == 8; }
 

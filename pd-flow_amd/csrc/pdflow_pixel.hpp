@@ -28,9 +28,15 @@ __device__ __forceinline__ float tau_w_of(float mu_uv, float lam, float acu_r)
 
 __device__ __forceinline__ void project(float &a, float &b)
 {
-    // unit-disc projection (.cu:849-859)
-    const float m = rhyp(a, b);
-    if (m < 1.f) { a = a * m; b = b * m; }
+    // unit-disc projection (.cu:849-859): m = 1/sqrt(a*a+b*b), scale when m < 1.  With
+    // correctly rounded sqrt and division, x = a*a+b*b <= 1 gives sqrt(x) <= 1 and so m >= 1:
+    // the square root and division are only evaluated when x > 1, with the same result for
+    // every input (NaN compares false either way).
+    const float x = a * a + b * b;
+    if (x > 1.f) {
+        const float m = 1.f / sqrtf(x);
+        if (m < 1.f) { a = a * m; b = b * m; }
+    }
 }
 
 __device__ __forceinline__ float clamp_pd(float pd)
@@ -78,5 +84,103 @@ __device__ __forceinline__ Primal primal_update(float du, float dv, float dw, fl
     return {du, dv, dw};
 }
 
+
+__device__ __forceinline__ float wdiff(float rn, float rp, float fn, float f0, float fp)
+{
+    // r-weighted central difference (.cu:593-595)
+    return (rn * (fn - f0) + rp * (f0 - fp)) / (rn + rp);
+}
+
+struct Tap {
+    unsigned su, fu, sv, fv;
+    float iu, iv;
+};
+
+__device__ __forceinline__ Tap make_tap(unsigned rows_i, unsigned cols_i, float iu, float iv)
+{
+    // clamp (.cu:656-659) and the four integer neighbours (.cu:661-664).  A NaN coordinate
+    // is taken as 0 so that no value can produce an index outside the plane (the
+    // reference leaves that case undefined); finite and infinite inputs are unaffected.
+    if (!(iu == iu)) iu = 0.f;
+    if (!(iv == iv)) iv = 0.f;
+    if (iu < 0.f) iu = 0.f;
+    else if (iu > (float)cols_i - 1.f) iu = (float)cols_i - 1.f;
+    if (iv < 0.f) iv = 0.f;
+    else if (iv > (float)rows_i - 1.f) iv = (float)rows_i - 1.f;
+    Tap t;
+    t.su = (unsigned)ceilf(iu);
+    t.fu = (unsigned)floorf(iu);
+    t.sv = (unsigned)ceilf(iv);
+    t.fv = (unsigned)floorf(iv);
+    t.iu = iu;
+    t.iv = iv;
+    return t;
+}
+
+__device__ __forceinline__ float bilinear(const float *mat, unsigned rows_i, const Tap &t)
+{
+    // interpolatePixel (.cu:666-681) after clamping
+    if (t.su == t.fu && t.sv == t.fv) return mat[(unsigned)rintf(t.iv + (float)rows_i * t.iu)];
+    if (t.su == t.fu) {
+        const unsigned cu = (unsigned)rintf(t.iu);
+        return ((float)t.sv - t.iv) * mat[t.fv + rows_i * cu] + (t.iv - (float)t.fv) * mat[t.sv + rows_i * cu];
+    }
+    if (t.sv == t.fv) {
+        const unsigned cv = (unsigned)rintf(t.iv);
+        return ((float)t.su - t.iu) * mat[cv + rows_i * t.fu] + (t.iu - (float)t.fu) * mat[cv + rows_i * t.su];
+    }
+    const float top = ((float)t.su - t.iu) * mat[t.sv + rows_i * t.fu] + (t.iu - (float)t.fu) * mat[t.sv + rows_i * t.su];
+    const float bot = ((float)t.su - t.iu) * mat[t.fv + rows_i * t.fu] + (t.iu - (float)t.fu) * mat[t.fv + rows_i * t.su];
+    return ((float)t.sv - t.iv) * bot + (t.iv - (float)t.fv) * top;
+}
+
+__device__ __forceinline__ float bilinear_depth(const float *mat, unsigned rows_i, const Tap &t)
+{
+    // interpolatePixelDepth (.cu:696-720): nearest neighbour (half to even) if a tap is 0
+    if (mat[t.sv + rows_i * t.su] == 0.f || mat[t.sv + rows_i * t.fu] == 0.f ||
+        mat[t.fv + rows_i * t.su] == 0.f || mat[t.fv + rows_i * t.fu] == 0.f) {
+        const unsigned ru = (unsigned)rintf(t.iu), rv = (unsigned)rintf(t.iv);
+        return mat[rv + rows_i * ru];
+    }
+    return bilinear(mat, rows_i, t);
+}
+
+// Stable sort of 9 (key, weight) pairs by key.  The reference uses a bubble sort with a
+// strict ">" (.cu:1457-1473); every stable sort yields the same permutation, so this
+// fully unrolled odd-even transposition network (same strict compare, registers only)
+// gives the same output.  Unused slots carry key +inf and weight 0 and stay at the end.
+__device__ __forceinline__ void sort9(float (&k)[9], float (&w)[9])
+{
+#pragma unroll
+    for (int round = 0; round < 9; round++) {
+#pragma unroll
+        for (int i = 1 + (round & 1); i < 9; i += 2) {
+            if (k[i - 1] > k[i]) {
+                const float tk = k[i - 1]; k[i - 1] = k[i]; k[i] = tk;
+                const float tw = w[i - 1]; w[i - 1] = w[i]; w[i] = tw;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ float wmedian_pick(const float (&fld)[9], const float (&cum)[9], float med)
+{
+    // .cu:1103-1110 with register-only indexing
+    unsigned cont = 0;
+    bool go = true;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        go = go && (med > cum[i]);
+        cont += go ? 1u : 0u;
+    }
+    if (cont == 0) return fld[0];
+    float cr = 0.f, cl = 0.f, fr = 0.f, fl = 0.f;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        if ((unsigned)i == cont) { cr = cum[i]; fr = fld[i]; }
+        if ((unsigned)i + 1u == cont) { cl = cum[i]; fl = fld[i]; }
+    }
+    return ((cr - med) * fl + (med - cl) * fr) / (cr - cl);
+}
 
 }  // namespace pdf

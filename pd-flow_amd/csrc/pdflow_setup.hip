@@ -1,0 +1,380 @@
+// Fused per-level set-up kernels of the solve path (pdf_solve): the reference's eleven
+// set-up launches per level (K2-K8, K13-K15; pdflow_cudalib.cu:439-808, :1018-1167) become four,
+// and the two frames' pyramids are built together.  Each output value is computed with the
+// same operations as the reference kernel that produces it, so results are bit-identical to
+// the stage-by-stage bridges (which remain available through the C-ABI).
+//
+//   k_level_init      K2 + K3 + K4: zero the level state and, above the coarsest level, filter
+//                     the previous level's solution up without materialising the zero-inserted
+//                     planes (a zero tap contributes g * 0.f exactly as in the reference).
+//   k_rij_gradients   K5 + K6: connectivity weights, then the weighted image gradients, which
+//                     need ri_2/rj_2 one pixel back: recomputed here instead of a barrier.
+//   k_warp_mu         K7 + K8: warping, then mu_uv (and, on request, the step-size planes).
+//   k_filter_motion   K13 + K14 + K15: the weighted median reads the saturated solution of its
+//                     3x3 neighbourhood, computed on the fly, and the motion field follows.
+//   k_pyramid2        K1 for two frames in one launch per level.
+#include <hip/hip_runtime.h>
+
+#include "pdflow_internal.hpp"
+#include "pdflow_pixel.hpp"
+
+namespace pdf {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ unsigned pixel_index() { return blockIdx.x * kBlock + threadIdx.x; }
+
+inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
+
+// ------------------------------------------------------------------ K2 + K3 + K4
+__constant__ int kInitZero[] = {PDF_DU_NEW, PDF_DV_NEW, PDF_DU_ACC, PDF_DV_ACC};
+__constant__ int kCarry[10] = {PDF_DU_L, PDF_DV_L, PDF_DW_L, PDF_PD_L, PDF_PUU_L,
+                               PDF_PUV_L, PDF_PVU_L, PDF_PVV_L, PDF_PWU_L, PDF_PWV_L};
+__constant__ int kUpDst[10] = {PDF_DU_PREV, PDF_DV_PREV, PDF_DW_NEW, PDF_PD, PDF_PUU,
+                               PDF_PUV, PDF_PVU, PDF_PVV, PDF_PWU, PDF_PWV};
+
+__global__ void __launch_bounds__(kBlock) k_level_init(LevelArgs a, int has_coarse)
+{
+    const unsigned index = pixel_index();
+    const unsigned rows_i = a.rows, cols_i = a.cols;
+    if (index >= rows_i * cols_i) return;
+    // .cu:447-454: du_new, dv_new, du_acc, dv_acc start at 0 on every level
+#pragma unroll
+    for (int k = 0; k < 4; k++) a.f[kInitZero[k]][index] = 0.f;
+    float acc[10];
+#pragma unroll
+    for (int q = 0; q < 10; q++) acc[q] = 0.f;
+    if (has_coarse) {
+        // .cu:460-569: the zero-inserted plane holds the coarse value at even (v, u) (du, dv
+        // doubled) and +0 elsewhere.  A running sum that starts at +0 is never -0 under
+        // round-to-nearest, so adding the +0 products of odd taps leaves it unchanged: only
+        // the even taps are visited, in the reference's (k outer, l inner) order.
+        const int v = (int)(index % rows_i), u = (int)(index / rows_i), rc = (int)rows_i / 2;
+        const bool inner = (v > 1) && (v < (int)rows_i - 2) && (u > 1) && (u < (int)cols_i - 2);
+        for (int k = (v & 1) ? -1 : -2; k < 3; k += 2) {
+            const int vv = v + k;
+            if (vv < 0 || vv >= (int)rows_i) continue;
+            for (int l = (u & 1) ? -1 : -2; l < 3; l += 2) {
+                const int uu = u + l;
+                if (uu < 0 || uu >= (int)cols_i) continue;
+                const float g = 4.f * a.g[12 + k + 5 * l];
+                const unsigned ci = (unsigned)(vv / 2) + (unsigned)(uu / 2) * (unsigned)rc;
+#pragma unroll
+                for (int q = 0; q < 10; q++) {
+                    const float cval = a.f[kCarry[q]][ci];
+                    acc[q] += g * (q < 2 ? 2.f * cval : cval);
+                }
+            }
+        }
+        if (!inner) {
+            // border weight: 1 - sum of g over the out-of-range taps (.cu:515-542)
+            float aw = 1.f;
+#pragma unroll
+            for (int k = -2; k < 3; k++)
+#pragma unroll
+                for (int l = -2; l < 3; l++) {
+                    const int vv = v + k, uu = u + l;
+                    if (vv < 0 || vv >= (int)rows_i || uu < 0 || uu >= (int)cols_i)
+                        aw -= 0.25f * (4.f * a.g[12 + k + 5 * l]);
+                }
+            const float inv = 1.f / aw;
+#pragma unroll
+            for (int q = 0; q < 10; q++) acc[q] *= inv;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 10; q++) a.f[kUpDst[q]][index] = acc[q];
+    a.f[PDF_DW_ACC][index] = acc[2];
+}
+
+// ------------------------------------------------------------------ K5 + K6
+__device__ __forceinline__ float rweight(float d1, float d2)
+{
+    return (fabsf(d1) + fabsf(d2) > 0.f) ? 2.f * rhyp(d1, d2) : 1.f;     // .cu:760-778
+}
+
+__global__ void __launch_bounds__(kBlock) k_rij_gradients(LevelArgs a)
+{
+    const unsigned index = pixel_index();
+    const unsigned rows_i = a.rows, cols_i = a.cols, r = rows_i;
+    if (index >= rows_i * cols_i) return;
+    const unsigned v = index % rows_i, u = index / rows_i;
+    // K5 (.cu:725-779)
+    float dxu = 0.f, dzu = 0.f, dxu2 = 0.f, dzu2 = 0.f;
+    float dyv = 0.f, dzv = 0.f, dyv2 = 0.f, dzv2 = 0.f;
+    if (u != cols_i - 1) {
+        dxu = a.Xold[index + r] - a.Xold[index];
+        dzu = a.Zold[index + r] - a.Zold[index];
+        dxu2 = a.X[index + r] - a.X[index];
+        dzu2 = a.Z[index + r] - a.Z[index];
+    }
+    if (v != rows_i - 1) {
+        dyv = a.Yold[index + 1] - a.Yold[index];
+        dzv = a.Zold[index + 1] - a.Zold[index];
+        dyv2 = a.Y[index + 1] - a.Y[index];
+        dzv2 = a.Z[index + 1] - a.Z[index];
+    }
+    const float ri2 = rweight(dxu2, dzu2), rj2 = rweight(dyv2, dzv2);
+    a.f[PDF_RI][index] = rweight(dxu, dzu);
+    a.f[PDF_RJ][index] = rweight(dyv, dzv);
+    a.f[PDF_RI_2][index] = ri2;
+    a.f[PDF_RJ_2][index] = rj2;
+    // K6 (.cu:574-627); ri_2 at (v, u-1) and rj_2 at (v-1, u) recomputed (u-1 and v-1 are
+    // never the last column/row there, so their differences always exist)
+    const float *I = a.I, *Z = a.Z;
+    float dcu, ddu, dcv, ddv;
+    if (u == 0) {
+        dcu = I[index + r] - I[index];
+        ddu = Z[index + r] - Z[index];
+    } else if (u == cols_i - 1) {
+        dcu = I[index] - I[index - r];
+        ddu = Z[index] - Z[index - r];
+    } else {
+        const float rp = rweight(a.X[index] - a.X[index - r], Z[index] - Z[index - r]);
+        dcu = wdiff(ri2, rp, I[index + r], I[index], I[index - r]);
+        ddu = (Z[index] > 0.f) ? wdiff(ri2, rp, Z[index + r], Z[index], Z[index - r]) : 0.f;
+    }
+    if (v == 0) {
+        dcv = I[index + 1] - I[index];
+        ddv = Z[index + 1] - Z[index];
+    } else if (v == rows_i - 1) {
+        dcv = I[index] - I[index - 1];
+        ddv = Z[index] - Z[index - 1];
+    } else {
+        const float rp = rweight(a.Y[index] - a.Y[index - 1], Z[index] - Z[index - 1]);
+        dcv = wdiff(rj2, rp, I[index + 1], I[index], I[index - 1]);
+        ddv = (Z[index] > 0.f) ? wdiff(rj2, rp, Z[index + 1], Z[index], Z[index - 1]) : 0.f;
+    }
+    a.f[PDF_DCU_AUX][index] = dcu;
+    a.f[PDF_DDU_AUX][index] = ddu;
+    a.f[PDF_DCV_AUX][index] = dcv;
+    a.f[PDF_DDV_AUX][index] = ddv;
+}
+
+// ------------------------------------------------------------------ K7 + K8
+__global__ void __launch_bounds__(kBlock) k_warp_mu(LevelArgs a, int write_steps)
+{
+    const unsigned index = pixel_index();
+    const unsigned rows_i = a.rows, cols_i = a.cols;
+    if (index >= rows_i * cols_i) return;
+    const unsigned v = index % rows_i, u = index / rows_i;
+    // K7 (.cu:629-652)
+    const float iu = (float)u + a.f[PDF_DU_PREV][index];
+    const float iv = (float)v + a.f[PDF_DV_PREV][index];
+    const Tap t = make_tap(rows_i, cols_i, iu, iv);
+    const float dct = bilinear(a.I, rows_i, t) - a.Iold[index];
+    const float dcu = bilinear(a.f[PDF_DCU_AUX], rows_i, t);
+    const float dcv = bilinear(a.f[PDF_DCV_AUX], rows_i, t);
+    const float wd = bilinear_depth(a.Z, rows_i, t);
+    const float ddt = (wd > 0.f) ? wd - a.Zold[index] : 0.f;
+    const float ddu = bilinear(a.f[PDF_DDU_AUX], rows_i, t);
+    const float ddv = bilinear(a.f[PDF_DDV_AUX], rows_i, t);
+    a.f[PDF_DCT][index] = dct;
+    a.f[PDF_DCU][index] = dcu;
+    a.f[PDF_DCV][index] = dcv;
+    a.f[PDF_DDT][index] = ddt;
+    a.f[PDF_DDU][index] = ddu;
+    a.f[PDF_DDV][index] = ddv;
+    // K8 computeMu (.cu:781-784)
+    const float mu_uv = a.mu / (1.f + 1000.f * (ddu * ddu + ddv * ddv + ddt * ddt));
+    a.f[PDF_MU_UV][index] = mu_uv;
+    if (write_steps) {
+        // K8 computeStepSizes (.cu:786-808), needed only by the unfused iteration bridges
+        const float ri = a.f[PDF_RI][index], rj = a.f[PDF_RJ][index];
+        a.f[PDF_SIGMA_PD][index] = sigma_pd_of(mu_uv, ddu, ddv);
+        a.f[PDF_SIGMA_PUVX][index] = sigma_p_of(a.lambda_i, ri);
+        a.f[PDF_SIGMA_PUVY][index] = sigma_p_of(a.lambda_i, rj);
+        a.f[PDF_SIGMA_PWX][index] = 0.5f / (ri * a.lambda_d + 1e-10f);
+        a.f[PDF_SIGMA_PWY][index] = 0.5f / (rj * a.lambda_d + 1e-10f);
+        float acu_r = ri + rj;
+        if (u > 0) acu_r += a.f[PDF_RI][index - rows_i];
+        if (v > 0) acu_r += a.f[PDF_RJ][index - 1];
+        a.f[PDF_TAU_U][index] = tau_uv_of(mu_uv, ddu, a.lambda_i, acu_r);
+        a.f[PDF_TAU_V][index] = tau_uv_of(mu_uv, ddv, a.lambda_i, acu_r);
+        a.f[PDF_TAU_W][index] = tau_w_of(mu_uv, a.lambda_d, acu_r);
+    }
+}
+
+// ------------------------------------------------------------------ K13 + K14 + K15
+struct Sat {
+    float du, dv, dw;
+};
+
+__device__ __forceinline__ Sat saturated(const LevelArgs &a, unsigned i)
+{
+    // saturateVariables (.cu:1018-1038) of pixel i
+    float du = a.f[PDF_DU_NEW][i], dv = a.f[PDF_DV_NEW][i], dw = a.f[PDF_DW_NEW][i];
+    if (du > 1.f) du = 1.f; else if (du < -1.f) du = -1.f;
+    if (dv > 1.f) dv = 1.f; else if (dv < -1.f) dv = -1.f;
+    if (a.Zold[i] == 0.f) dw = 0.f;
+    return {du + a.f[PDF_DU_PREV][i], dv + a.f[PDF_DV_PREV][i], dw};
+}
+
+__global__ void __launch_bounds__(kBlock) k_filter_motion(LevelArgs a)
+{
+    const unsigned index = pixel_index();
+    const unsigned rows_i = a.rows, cols_i = a.cols;
+    if (index >= rows_i * cols_i) return;
+    const unsigned v = index % rows_i, u = index / rows_i;
+    const float *zo = a.Zold;
+    const float d0 = zo[index];
+    float ou, ov, ow;
+    if (d0 > 0.f) {
+        // .cu:1057-1132: taps in the reference's order (k = v offset outer, l = u offset inner)
+        float ku[9], kv[9], kw[9], wu[9], wv[9], ww[9];
+        int n = 0;
+#pragma unroll
+        for (int s = 0; s < 9; s++) {
+            ku[s] = kv[s] = kw[s] = __builtin_inff();
+            wu[s] = wv[s] = ww[s] = 0.f;
+        }
+#pragma unroll
+        for (int k = -1; k < 2; k++)
+#pragma unroll
+            for (int l = -1; l < 2; l++) {
+                const int r = (int)v + k, c = (int)u + l;
+                const bool ok = !(r < 0 || r >= (int)rows_i || c < 0 || c >= (int)cols_i);
+                const unsigned il = ok ? index + l * (int)rows_i + k : index;
+                const float dz = d0 - zo[il], dt = a.f[PDF_DDT][il];
+                const float pres = 1.f / (1.f + 5.f * (dz * dz) + 10.f * (dt * dt));
+                const Sat f = saturated(a, il);
+#pragma unroll
+                for (int s = 0; s < 9; s++)
+                    if (ok && s == n) {
+                        ku[s] = f.du; kv[s] = f.dv; kw[s] = f.dw;
+                        wu[s] = pres; wv[s] = pres; ww[s] = pres;
+                    }
+                n += ok ? 1 : 0;
+            }
+        sort9(ku, wu);
+        sort9(kv, wv);
+        sort9(kw, ww);
+        float cu[9], cv[9], cw[9];
+        cu[0] = wu[0]; cv[0] = wv[0]; cw[0] = ww[0];
+#pragma unroll
+        for (int i = 1; i < 9; i++) {
+            cu[i] = cu[i - 1] + wu[i];
+            cv[i] = cv[i - 1] + wv[i];
+            cw[i] = cw[i - 1] + ww[i];
+        }
+        const float med = 0.5f * cu[8];
+        ou = wmedian_pick(ku, cu, med);
+        ov = wmedian_pick(kv, cv, med);
+        ow = wmedian_pick(kw, cw, med);
+    } else {
+        const Sat f = saturated(a, index);
+        ou = f.du; ov = f.dv; ow = f.dw;
+    }
+    a.f[PDF_DU_L][index] = ou;
+    a.f[PDF_DV_L][index] = ov;
+    a.f[PDF_DW_L][index] = ow;
+    a.f[PDF_PD_L][index] = a.f[PDF_PD][index];
+    a.f[PDF_PUU_L][index] = a.f[PDF_PUU][index];
+    a.f[PDF_PUV_L][index] = a.f[PDF_PUV][index];
+    a.f[PDF_PVU_L][index] = a.f[PDF_PVU][index];
+    a.f[PDF_PVV_L][index] = a.f[PDF_PVV][index];
+    a.f[PDF_PWU_L][index] = a.f[PDF_PWU][index];
+    a.f[PDF_PWV_L][index] = a.f[PDF_PWV][index];
+    // K15 (.cu:1150-1167)
+    if (d0 > 0.f) {
+        a.f[PDF_DX][index] = ow;
+        a.f[PDF_DY][index] = d0 * ou * a.inv_f + ow * a.Xold[index] / d0;
+        a.f[PDF_DZ][index] = d0 * ov * a.inv_f + ow * a.Yold[index] / d0;
+    } else {
+        a.f[PDF_DX][index] = 0.f;
+        a.f[PDF_DY][index] = 0.f;
+        a.f[PDF_DZ][index] = 0.f;
+    }
+}
+
+// ------------------------------------------------------------------ K1 for two frames
+__global__ void __launch_bounds__(kBlock) k_pyramid2(Pyr2Args a)
+{
+    const PyrArgs &p = a.frame[blockIdx.y];
+    const unsigned index = pixel_index();
+    const unsigned rows_i = p.rows_i, cols_i = p.cols_i;
+    if (index >= rows_i * cols_i) return;
+    const unsigned v = index % rows_i, u = index / rows_i;
+    const float max_depth_dif = 0.1f;
+    float depth;
+    if (p.level == 0) {
+        p.col[index] = p.colour_wf[index];
+        depth = p.depth_wf[index];
+        p.dep[index] = depth;
+    } else {
+        // .cu:357-424
+        float sumd = 0.f, sumc = 0.f, wd = 0.f, wc = 0.f;
+        const float dcen = p.depp[2 * v + 4 * u * rows_i];
+        const bool inner = (v > 0) && (v < rows_i - 1) && (u > 0) && (u < cols_i - 1);
+#pragma unroll
+        for (int k = -2; k < 3; k++)
+#pragma unroll
+            for (int l = -2; l < 3; l++) {
+                const int vv = 2 * (int)v + k, uu = 2 * (int)u + l;
+                if (!inner && !(vv >= 0 && vv < 2 * (int)rows_i && uu >= 0 && uu < 2 * (int)cols_i))
+                    continue;
+                const unsigned ip = (unsigned)vv + (unsigned)uu * 2 * rows_i;
+                const float g = p.g[12 + k + 5 * l];
+                sumc += g * p.colp[ip];
+                if (!inner) wc += g;
+                const float d = p.depp[ip];
+                if ((d > 0.f) && (fabsf(d - dcen) < max_depth_dif)) {
+                    const float w = g * (max_depth_dif - fabsf(d - dcen));
+                    wd += w;
+                    sumd += w * d;
+                }
+            }
+        depth = (sumd > 0.f) ? sumd / wd : 0.f;
+        p.dep[index] = depth;
+        p.col[index] = inner ? sumc : sumc / wc;
+    }
+    const float disp_u = 0.5f * (float)(cols_i - 1);
+    const float disp_v = 0.5f * (float)(rows_i - 1);
+    p.xx[index] = ((float)u - disp_u) * depth * p.inv_f_i;
+    p.yy[index] = ((float)v - disp_v) * depth * p.inv_f_i;
+}
+
+}  // namespace
+
+hipError_t launch_level_init(const LevelArgs &a, bool has_coarse, hipStream_t s)
+{
+    const unsigned n = a.rows * a.cols;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_level_init, grid_for(n), dim3(kBlock), 0, s, a, has_coarse ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_rij_gradients(const LevelArgs &a, hipStream_t s)
+{
+    const unsigned n = a.rows * a.cols;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rij_gradients, grid_for(n), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_warp_mu(const LevelArgs &a, bool write_steps, hipStream_t s)
+{
+    const unsigned n = a.rows * a.cols;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_warp_mu, grid_for(n), dim3(kBlock), 0, s, a, write_steps ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_motion(const LevelArgs &a, hipStream_t s)
+{
+    const unsigned n = a.rows * a.cols;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_filter_motion, grid_for(n), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pyramid2(const Pyr2Args &a, hipStream_t s)
+{
+    const unsigned n = a.frame[0].rows_i * a.frame[0].cols_i;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pyramid2, dim3((n + kBlock - 1) / kBlock, 2), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace pdf

@@ -78,6 +78,7 @@ def hip_library():
         "pdf_free_level": ([vp], c_int),
         "pdf_pd_iterations": ([vp, c_uint], c_int),
         "pdf_set_iteration_kernel": ([vp, c_int, c_uint], c_int),
+        "pdf_set_solve_options": ([vp, c_int, c_int], c_int),
         "pdf_copy_motion_field": ([vp, fp, fp, fp], c_int),
         "pdf_copy_all_solutions": ([vp] + [fp] * 11, c_int),
         "pdf_motion_field_device": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)], c_int),
@@ -226,6 +227,9 @@ class Solver:
         """k iterations per launch (temporal blocking) on levels of >= min_pixels pixels."""
         self._chk(self.lib.pdf_set_iteration_kernel(self.ctx, k, min_pixels))
 
+    def set_solve_options(self, fused_setup=True, level_kernel=True):
+        self._chk(self.lib.pdf_set_solve_options(self.ctx, int(fused_setup), int(level_kernel)))
+
     def solve(self, iters=None, graph=False):
         arr = None if iters is None else (ctypes.c_uint * len(iters))(*iters)
         fn = self.lib.pdf_solve_graph if graph else self.lib.pdf_solve
@@ -262,6 +266,12 @@ class Solver:
         dx, dy, dz = (np.empty(r * c, np.float32) for _ in range(3))
         self._chk(self.lib.pdf_copy_motion_field(self.ctx, _fptr(dx), _fptr(dy), _fptr(dz)))
         return dx, dy, dz
+
+    def solve_pair_device(self, bufs, iters=None, graph=False):
+        """One pair from four device buffers (I1, Z1, I2, Z2); flow stays on the device."""
+        arr = None if iters is None else (ctypes.c_uint * len(iters))(*iters)
+        self._chk(self.lib.pdf_solve_pair_device(self.ctx, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr,
+                                                 bufs[3].ptr, arr, 1 if graph else 0))
 
     def solve_pair(self, I1, Z1, I2, Z2, iters=None, graph=False):
         """The pair evaluator end to end from host planes: returns (dx, dy, dz)."""

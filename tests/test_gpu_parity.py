@@ -227,3 +227,41 @@ def test_temporal_blocking_matches_oracle(gpu_available, k):
             s.stage("motion_field")
     for a, b, n in zip(g.motion_field(), o.motion_field(), ("dx", "dy", "dz")):
         assert_same(a, b, n)
+
+
+@pytest.mark.parametrize("fused_setup,level_kernel", [(True, True), (False, True), (True, False), (False, False)])
+def test_solve_options_agree_bit_for_bit(gpu_available, fused_setup, level_kernel):
+    """Fused set-up kernels and the whole-level kernel change launch structure only."""
+    say(f"solve options fused_setup={fused_setup} level_kernel={level_kernel}")
+    I1, Z1, I2, Z2 = synth_pair_float(55, 240, 320)
+    ref = Solver(240, 2)
+    ref.set_solve_options(False, False)
+    ref.set_iteration_kernel(1, 0)
+    want = ref.solve_pair(I1, Z1, I2, Z2)
+    g = Solver(240, 2)
+    g.set_solve_options(fused_setup, level_kernel)
+    got = g.solve_pair(I1, Z1, I2, Z2)
+    for a, b in zip(got, want):
+        np.testing.assert_array_equal(a, b)
+    # carry planes of the finest level agree too
+    for f in ("du_l", "dv_l", "dw_l", "pd_l", "puu_l", "pwv_l"):
+        np.testing.assert_array_equal(g.get_field(f), ref.get_field(f))
+
+
+def test_device_pair_path_matches_host_path(gpu_available):
+    """pdf_solve_pair_device (two-frame pyramid from device planes) equals the reference
+    ingest sequence from host planes, with and without the graph."""
+    say("device pair path")
+    from pdflow.lib import DeviceBuffer
+    I1, Z1, I2, Z2 = synth_pair_float(66, 480, 640)
+    g = Solver(480, 1)
+    want = g.solve_pair(I1, Z1, I2, Z2)
+    bufs = []
+    for arr in (I1, Z1, I2, Z2):
+        b = DeviceBuffer(arr.nbytes)
+        b.upload(arr)
+        bufs.append(b)
+    for graph in (False, True, True):
+        g.solve_pair_device(bufs, graph=graph)
+        for a, b in zip(g.motion_field(), want):
+            np.testing.assert_array_equal(a, b)
