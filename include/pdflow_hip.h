@@ -159,6 +159,42 @@ int pdf_enable_kernel_timing(pdf_ctx *ctx, int enable);
 int pdf_kernel_timing(pdf_ctx *ctx, uint64_t *launches, double *total_ms,
                       uint64_t *pixel_iters);
 
+/* ---- batched ragged solve (no reference counterpart: the reference solves one pair per
+ * CSF_cuda object) -------------------------------------------------------------------- */
+
+typedef struct pdf_batch pdf_batch;
+
+/* A batch of npairs independent pairs.  Pair i is solved at rows[i] x rows[i]*4/3 from input
+ * frames of (640/cam_mode[i]) x (480/cam_mode[i]) (cam_mode NULL: 480/rows[i], the input at
+ * the solve size), with the reference schedule for its level count (num_max_iter NULL) or
+ * num_max_iter[i*6 + level], coarse first.  lambda_i, lambda_d, mu, the mask and fovh are
+ * shared (readParameters, pdflow_cudalib.cu:176-193).  All memory is allocated here. */
+int pdf_batch_create(pdf_batch **out, unsigned npairs, const unsigned *rows, const unsigned *cam_mode,
+                     float lambda_i, float lambda_d, float mu, const float *g_mask25, float fovh,
+                     const unsigned *num_max_iter);
+int pdf_batch_destroy(pdf_batch *b);
+int pdf_batch_size(const pdf_batch *b, unsigned *npairs);
+const char *pdf_batch_last_error(const pdf_batch *b);   /* NULL b: last create failure */
+/* iteration kernels for the batch (as pdf_set_iteration_kernel; level_kernel as in
+ * pdf_set_solve_options); invalidates the recorded plan and graph */
+int pdf_batch_set_iteration_kernel(pdf_batch *b, int k, unsigned min_pixels, int level_kernel);
+/* device pointers of pair's four input planes (the caller may fill them directly) */
+int pdf_batch_inputs_device(pdf_batch *b, unsigned pair, float **I1, float **Z1, float **I2,
+                            float **Z2);
+/* blocking upload of one pair's frames from host planes */
+int pdf_batch_set_inputs(pdf_batch *b, unsigned pair, const float *I1, const float *Z1,
+                         const float *I2, const float *Z2);
+/* solves every pair (two pyramids + coarse-to-fine solve each); asynchronous on the batch's
+ * stream; use_graph replays a HIP graph captured on first use */
+int pdf_batch_solve(pdf_batch *b, int use_graph);
+int pdf_batch_synchronize(pdf_batch *b);
+/* pair's finest-level flow planes (rows x cols, column-major) on the device / copied out */
+int pdf_batch_flow_device(pdf_batch *b, unsigned pair, const float **dx, const float **dy,
+                          const float **dz, unsigned *rows, unsigned *cols);
+int pdf_batch_copy_flow(pdf_batch *b, unsigned pair, float *dx, float *dy, float *dz);
+/* number of kernel launches one batched solve makes */
+int pdf_batch_launch_count(const pdf_batch *b, unsigned *launches);
+
 /* ---- device memory helpers for callers without a HIP binding ------------------------ */
 int pdf_device_alloc(void **ptr, size_t bytes);
 int pdf_device_free(void *ptr);

@@ -48,7 +48,36 @@ struct PDArgs {
     float *out[S_NUM];
     unsigned rows, cols;
     float lambda_i, lambda_d;
+    unsigned iters;          // iterations, for the batched whole-level kernel
 };
+
+// A batched launch: items (pairs, or pair-frames for the pyramid) each cover a contiguous
+// range of the grid's blocks.  args points to a device array of per-item argument structs
+// (LevelArgs, PDArgs or PyrArgs); first[i] is item i's first block, first[n] the total.
+struct BatchTable {
+    const void *args;
+    const unsigned *first;
+    unsigned n;
+};
+
+struct BatchItem {
+    unsigned item, block;
+};
+
+// the item that owns this block: largest i with first[i] <= blockIdx.x (binary search)
+__device__ __forceinline__ BatchItem batch_item(const BatchTable &t)
+{
+    const unsigned b = blockIdx.x;
+    unsigned lo = 0, hi = t.n;            // invariant: first[lo] <= b < first[hi]
+    while (hi - lo > 1) {
+        const unsigned mid = (lo + hi) / 2;
+        if (t.first[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return {lo, b - t.first[lo]};
+}
+
+enum class BatchKernel { LevelInit, RijGradients, WarpMu, FilterMotion, Pyramid };
 
 // Iterations per launch supported by the temporally blocked kernel.
 constexpr int kTBMaxK = 8;
@@ -78,6 +107,12 @@ hipError_t launch_rij_gradients(const LevelArgs &a, hipStream_t s);
 hipError_t launch_warp_mu(const LevelArgs &a, bool write_steps, hipStream_t s);
 hipError_t launch_filter_motion(const LevelArgs &a, hipStream_t s);
 hipError_t launch_pyramid2(const Pyr2Args &a, hipStream_t s);
+// batched launches (flag: has_coarse for LevelInit, unused otherwise)
+hipError_t launch_batch(BatchKernel k, const BatchTable &t, unsigned total_blocks, int flag, hipStream_t s);
+unsigned batch_blocks_per_level(unsigned pixels);
+// k = 0: whole-level kernel (iters from PDArgs), 1: one iteration, >1: temporal blocking
+hipError_t launch_pd_batch(int k, const BatchTable &t, unsigned total_blocks, hipStream_t s);
+unsigned pd_blocks(int k, unsigned rows, unsigned cols);
 hipError_t launch_motion_field(const LevelArgs &a, hipStream_t s);
 
 }  // namespace pdf

@@ -33,7 +33,7 @@ struct TVDual {
     float puu, puv, pvu, pvv, pwu, pwv;
 };
 
-__global__ void __launch_bounds__(TV * TU) k_pd_fused(PDArgs a)
+__device__ __forceinline__ void pd_fused_body(const PDArgs &a, unsigned bx, unsigned by)
 {
     __shared__ float sU[RU * RV], sV[RU * RV], sW[RU * RV];
     __shared__ float sRi[HU * HV], sRj[HU * HV];
@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(TV * TU) k_pd_fused(PDArgs a)
 
     const int tid = threadIdx.x;
     const int tv = tid % TV, tu = tid / TV;
-    const int v0 = blockIdx.x * TV, u0 = blockIdx.y * TU;
+    const int v0 = (int)bx * TV, u0 = (int)by * TU;
     const int rows = (int)a.rows, cols = (int)a.cols;
     const float li = a.lambda_i, ld = a.lambda_d;
     const float *du_acc = a.in[S_DU_ACC], *dv_acc = a.in[S_DV_ACC], *dw_acc = a.in[S_DW_ACC];
@@ -302,7 +302,7 @@ constexpr int BT = 1024;               // threads per workgroup, two pixels (col
 #define PD_BOTH(f) do { f(x0, 0); f(x1, 1); } while (0)
 
 template <int K>
-__global__ void __launch_bounds__(BT) k_pd_tb(PDArgs a)
+__device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned by)
 {
     constexpr int TVk = BV - 2 * K, TUk = BU - 2 * K;
     __shared__ float sU[BU * BV], sV[BU * BV], sW[BU * BV];
@@ -310,7 +310,7 @@ __global__ void __launch_bounds__(BT) k_pd_tb(PDArgs a)
     __shared__ float sRi[BU * BV], sRj[BU * BV];
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int rv0 = (int)blockIdx.x * TVk - K, ru0 = (int)blockIdx.y * TUk - K;  // region origin
+    const int rv0 = (int)bx * TVk - K, ru0 = (int)by * TUk - K;  // region origin
     const int rows = (int)a.rows, cols = (int)a.cols;
     const float li = a.lambda_i, ld = a.lambda_d;
     const int lv = lane;
@@ -379,7 +379,7 @@ __global__ void __launch_bounds__(BT) k_pd_tb(PDArgs a)
 // the image borders are the only borders.
 constexpr int kLevelMaxPixels = 2 * BT;
 
-__global__ void __launch_bounds__(BT) k_pd_level(PDArgs a, int n)
+__device__ __forceinline__ void level_body(const PDArgs &a, int n)
 {
     __shared__ float sU[kLevelMaxPixels], sV[kLevelMaxPixels], sW[kLevelMaxPixels];
     __shared__ float sP[6][kLevelMaxPixels];
@@ -446,6 +446,35 @@ __global__ void __launch_bounds__(BT) k_pd_level(PDArgs a, int n)
     PD_BOTH(store);
 }
 
+// ------------------------------------------------------------------ entry points
+__global__ void __launch_bounds__(TV * TU) k_pd_fused(PDArgs a) { pd_fused_body(a, blockIdx.x, blockIdx.y); }
+template <int K>
+__global__ void __launch_bounds__(BT) k_pd_tb(PDArgs a) { tb_body<K>(a, blockIdx.x, blockIdx.y); }
+__global__ void __launch_bounds__(BT) k_pd_level(PDArgs a, int n) { level_body(a, n); }
+
+// batched: every table entry is one pair's PDArgs; tiles of a pair are numbered v-major
+__global__ void __launch_bounds__(TV * TU) kb_pd_fused(BatchTable t)
+{
+    const BatchItem it = batch_item(t);
+    const PDArgs &a = static_cast<const PDArgs *>(t.args)[it.item];
+    const unsigned tiles_v = (a.rows + TV - 1) / TV;
+    pd_fused_body(a, it.block % tiles_v, it.block / tiles_v);
+}
+template <int K>
+__global__ void __launch_bounds__(BT) kb_pd_tb(BatchTable t)
+{
+    const BatchItem it = batch_item(t);
+    const PDArgs &a = static_cast<const PDArgs *>(t.args)[it.item];
+    const unsigned tiles_v = (a.rows + (BV - 2 * K) - 1) / (BV - 2 * K);
+    tb_body<K>(a, it.block % tiles_v, it.block / tiles_v);
+}
+__global__ void __launch_bounds__(BT) kb_pd_level(BatchTable t)
+{
+    const BatchItem it = batch_item(t);
+    const PDArgs &a = static_cast<const PDArgs *>(t.args)[it.item];
+    level_body(a, (int)a.iters);
+}
+
 }  // namespace
 
 hipError_t launch_pd_fused(const PDArgs &a, hipStream_t s)
@@ -488,6 +517,35 @@ hipError_t launch_pd_tb(const PDArgs &a, int k, hipStream_t s)
     case 8: return launch_tb<8>(a, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+}  // namespace pdf
+
+namespace pdf {
+
+unsigned pd_blocks(int k, unsigned rows, unsigned cols)
+{
+    if (k <= 0) return 1;                                   // whole-level kernel
+    if (k == 1) return ((rows + TV - 1) / TV) * ((cols + TU - 1) / TU);
+    const unsigned tv = BV - 2 * k, tu = BU - 2 * k;
+    return ((rows + tv - 1) / tv) * ((cols + tu - 1) / tu);
+}
+
+hipError_t launch_pd_batch(int k, const BatchTable &t, unsigned total_blocks, hipStream_t s)
+{
+    if (total_blocks == 0 || t.n == 0) return hipSuccess;
+    const dim3 grid(total_blocks);
+    switch (k) {
+    case 0: hipLaunchKernelGGL(kb_pd_level, grid, dim3(BT), 0, s, t); break;
+    case 1: hipLaunchKernelGGL(kb_pd_fused, grid, dim3(TV * TU), 0, s, t); break;
+    case 2: hipLaunchKernelGGL(kb_pd_tb<2>, grid, dim3(BT), 0, s, t); break;
+    case 3: hipLaunchKernelGGL(kb_pd_tb<3>, grid, dim3(BT), 0, s, t); break;
+    case 4: hipLaunchKernelGGL(kb_pd_tb<4>, grid, dim3(BT), 0, s, t); break;
+    case 6: hipLaunchKernelGGL(kb_pd_tb<6>, grid, dim3(BT), 0, s, t); break;
+    case 8: hipLaunchKernelGGL(kb_pd_tb<8>, grid, dim3(BT), 0, s, t); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 }  // namespace pdf

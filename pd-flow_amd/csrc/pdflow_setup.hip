@@ -23,7 +23,6 @@ namespace {
 
 constexpr int kBlock = 256;
 
-__device__ __forceinline__ unsigned pixel_index() { return blockIdx.x * kBlock + threadIdx.x; }
 
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 
@@ -34,9 +33,9 @@ __constant__ int kCarry[10] = {PDF_DU_L, PDF_DV_L, PDF_DW_L, PDF_PD_L, PDF_PUU_L
 __constant__ int kUpDst[10] = {PDF_DU_PREV, PDF_DV_PREV, PDF_DW_NEW, PDF_PD, PDF_PUU,
                                PDF_PUV, PDF_PVU, PDF_PVV, PDF_PWU, PDF_PWV};
 
-__global__ void __launch_bounds__(kBlock) k_level_init(LevelArgs a, int has_coarse)
+__device__ __forceinline__ void level_init_body(const LevelArgs &a, unsigned block, int has_coarse)
 {
-    const unsigned index = pixel_index();
+    const unsigned index = block * kBlock + threadIdx.x;
     const unsigned rows_i = a.rows, cols_i = a.cols;
     if (index >= rows_i * cols_i) return;
     // .cu:447-454: du_new, dv_new, du_acc, dv_acc start at 0 on every level
@@ -94,9 +93,9 @@ __device__ __forceinline__ float rweight(float d1, float d2)
     return (fabsf(d1) + fabsf(d2) > 0.f) ? 2.f * rhyp(d1, d2) : 1.f;     // .cu:760-778
 }
 
-__global__ void __launch_bounds__(kBlock) k_rij_gradients(LevelArgs a)
+__device__ __forceinline__ void rij_gradients_body(const LevelArgs &a, unsigned block)
 {
-    const unsigned index = pixel_index();
+    const unsigned index = block * kBlock + threadIdx.x;
     const unsigned rows_i = a.rows, cols_i = a.cols, r = rows_i;
     if (index >= rows_i * cols_i) return;
     const unsigned v = index % rows_i, u = index / rows_i;
@@ -153,9 +152,9 @@ __global__ void __launch_bounds__(kBlock) k_rij_gradients(LevelArgs a)
 }
 
 // ------------------------------------------------------------------ K7 + K8
-__global__ void __launch_bounds__(kBlock) k_warp_mu(LevelArgs a, int write_steps)
+__device__ __forceinline__ void warp_mu_body(const LevelArgs &a, unsigned block, int write_steps)
 {
-    const unsigned index = pixel_index();
+    const unsigned index = block * kBlock + threadIdx.x;
     const unsigned rows_i = a.rows, cols_i = a.cols;
     if (index >= rows_i * cols_i) return;
     const unsigned v = index % rows_i, u = index / rows_i;
@@ -211,9 +210,9 @@ __device__ __forceinline__ Sat saturated(const LevelArgs &a, unsigned i)
     return {du + a.f[PDF_DU_PREV][i], dv + a.f[PDF_DV_PREV][i], dw};
 }
 
-__global__ void __launch_bounds__(kBlock) k_filter_motion(LevelArgs a)
+__device__ __forceinline__ void filter_motion_body(const LevelArgs &a, unsigned block)
 {
-    const unsigned index = pixel_index();
+    const unsigned index = block * kBlock + threadIdx.x;
     const unsigned rows_i = a.rows, cols_i = a.cols;
     if (index >= rows_i * cols_i) return;
     const unsigned v = index % rows_i, u = index / rows_i;
@@ -289,10 +288,9 @@ __global__ void __launch_bounds__(kBlock) k_filter_motion(LevelArgs a)
 }
 
 // ------------------------------------------------------------------ K1 for two frames
-__global__ void __launch_bounds__(kBlock) k_pyramid2(Pyr2Args a)
+__device__ __forceinline__ void pyramid_body(const PyrArgs &p, unsigned block)
 {
-    const PyrArgs &p = a.frame[blockIdx.y];
-    const unsigned index = pixel_index();
+    const unsigned index = block * kBlock + threadIdx.x;
     const unsigned rows_i = p.rows_i, cols_i = p.cols_i;
     if (index >= rows_i * cols_i) return;
     const unsigned v = index % rows_i, u = index / rows_i;
@@ -333,6 +331,41 @@ __global__ void __launch_bounds__(kBlock) k_pyramid2(Pyr2Args a)
     const float disp_v = 0.5f * (float)(rows_i - 1);
     p.xx[index] = ((float)u - disp_u) * depth * p.inv_f_i;
     p.yy[index] = ((float)v - disp_v) * depth * p.inv_f_i;
+}
+
+// ------------------------------------------------------------------ entry points
+// Single level: arguments by value.  Batched: one launch over many pairs, each block finds
+// its pair's arguments in a device table (see BatchTable).
+__global__ void __launch_bounds__(kBlock) k_level_init(LevelArgs a, int has_coarse) { level_init_body(a, blockIdx.x, has_coarse); }
+__global__ void __launch_bounds__(kBlock) k_rij_gradients(LevelArgs a) { rij_gradients_body(a, blockIdx.x); }
+__global__ void __launch_bounds__(kBlock) k_warp_mu(LevelArgs a, int write_steps) { warp_mu_body(a, blockIdx.x, write_steps); }
+__global__ void __launch_bounds__(kBlock) k_filter_motion(LevelArgs a) { filter_motion_body(a, blockIdx.x); }
+__global__ void __launch_bounds__(kBlock) k_pyramid2(Pyr2Args a) { pyramid_body(a.frame[blockIdx.y], blockIdx.x); }
+
+__global__ void __launch_bounds__(kBlock) kb_level_init(BatchTable t, int has_coarse)
+{
+    const BatchItem it = batch_item(t);
+    level_init_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, has_coarse);
+}
+__global__ void __launch_bounds__(kBlock) kb_rij_gradients(BatchTable t)
+{
+    const BatchItem it = batch_item(t);
+    rij_gradients_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block);
+}
+__global__ void __launch_bounds__(kBlock) kb_warp_mu(BatchTable t)
+{
+    const BatchItem it = batch_item(t);
+    warp_mu_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, 0);
+}
+__global__ void __launch_bounds__(kBlock) kb_filter_motion(BatchTable t)
+{
+    const BatchItem it = batch_item(t);
+    filter_motion_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block);
+}
+__global__ void __launch_bounds__(kBlock) kb_pyramid(BatchTable t)
+{
+    const BatchItem it = batch_item(t);
+    pyramid_body(static_cast<const PyrArgs *>(t.args)[it.item], it.block);
 }
 
 }  // namespace
@@ -376,5 +409,27 @@ hipError_t launch_pyramid2(const Pyr2Args &a, hipStream_t s)
     hipLaunchKernelGGL(k_pyramid2, dim3((n + kBlock - 1) / kBlock, 2), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
+
+}  // namespace pdf
+
+namespace pdf {
+
+hipError_t launch_batch(BatchKernel k, const BatchTable &t, unsigned total_blocks, int flag,
+                        hipStream_t s)
+{
+    if (total_blocks == 0 || t.n == 0) return hipSuccess;
+    const dim3 grid(total_blocks), block(kBlock);
+    switch (k) {
+    case BatchKernel::LevelInit: hipLaunchKernelGGL(kb_level_init, grid, block, 0, s, t, flag); break;
+    case BatchKernel::RijGradients: hipLaunchKernelGGL(kb_rij_gradients, grid, block, 0, s, t); break;
+    case BatchKernel::WarpMu: hipLaunchKernelGGL(kb_warp_mu, grid, block, 0, s, t); break;
+    case BatchKernel::FilterMotion: hipLaunchKernelGGL(kb_filter_motion, grid, block, 0, s, t); break;
+    case BatchKernel::Pyramid: hipLaunchKernelGGL(kb_pyramid, grid, block, 0, s, t); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+unsigned batch_blocks_per_level(unsigned pixels) { return (pixels + kBlock - 1) / kBlock; }
 
 }  // namespace pdf

@@ -8,6 +8,7 @@ there is no CPU fallback.
 """
 from .lib import (  # noqa: F401
     FIELDS,
+    Batch,
     PYR_KINDS,
     PDFlowError,
     Solver,

@@ -99,6 +99,22 @@ def hip_library():
         "pdf_memcpy_d2h": ([vp, vp, ctypes.c_size_t], c_int),
         "pdf_device_count": ([ctypes.POINTER(c_int)], c_int),
     }
+    up_ = ctypes.POINTER(ctypes.c_uint)
+    fpp = ctypes.POINTER(ctypes.POINTER(ctypes.c_float))
+    sigs.update({
+        "pdf_batch_create": ([ctypes.POINTER(vp), c_uint, up_, up_, c_float, c_float, c_float, fp, c_float, up_], c_int),
+        "pdf_batch_destroy": ([vp], c_int),
+        "pdf_batch_size": ([vp, up_], c_int),
+        "pdf_batch_last_error": ([vp], ctypes.c_char_p),
+        "pdf_batch_set_iteration_kernel": ([vp, c_int, c_uint, c_int], c_int),
+        "pdf_batch_inputs_device": ([vp, c_uint, fpp, fpp, fpp, fpp], c_int),
+        "pdf_batch_set_inputs": ([vp, c_uint, fp, fp, fp, fp], c_int),
+        "pdf_batch_solve": ([vp, c_int], c_int),
+        "pdf_batch_synchronize": ([vp], c_int),
+        "pdf_batch_flow_device": ([vp, c_uint, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), up_, up_], c_int),
+        "pdf_batch_copy_flow": ([vp, c_uint, fp, fp, fp], c_int),
+        "pdf_batch_launch_count": ([vp, up_], c_int),
+    })
     for stage in ("assign_zeros", "upsample", "rij", "image_gradients", "warping",
                   "mu_and_step_sizes", "gradient", "dual_variables", "divergence",
                   "primal_variables", "filter", "motion_field"):
@@ -289,3 +305,69 @@ class Solver:
         n, ms, pix = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_uint64()
         self._chk(self.lib.pdf_kernel_timing(self.ctx, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(pix)))
         return n.value, ms.value, pix.value
+
+
+class Batch:
+    """A batched ragged solve: many pairs of mixed size solved by shared launches."""
+
+    def __init__(self, rows_list, iters=None, cam_modes=None, **overrides):
+        self.lib = hip_library()
+        p = default_params(480, 1)
+        p.update(overrides)
+        self.rows = list(rows_list)
+        n = len(self.rows)
+        rows_arr = (ctypes.c_uint * n)(*self.rows)
+        cam_arr = None if cam_modes is None else (ctypes.c_uint * n)(*cam_modes)
+        self.cam_modes = list(cam_modes) if cam_modes is not None else [480 // r for r in self.rows]
+        it_arr = None
+        if iters is not None:
+            flat = []
+            for sched in iters:
+                flat += list(sched) + [0] * (6 - len(sched))
+            it_arr = (ctypes.c_uint * len(flat))(*flat)
+        mask = np.ascontiguousarray(p["g_mask"], np.float32)
+        self.ctx = ctypes.c_void_p()
+        rc = self.lib.pdf_batch_create(ctypes.byref(self.ctx), n, rows_arr, cam_arr, p["lambda_i"],
+                                       p["lambda_d"], p["mu"], _fptr(mask), p["fovh"], it_arr)
+        if rc:
+            raise PDFlowError(rc, self.lib.pdf_batch_last_error(None).decode())
+
+    def _chk(self, rc):
+        if rc:
+            raise PDFlowError(rc, self.lib.pdf_batch_last_error(self.ctx).decode())
+
+    def close(self):
+        if self.ctx:
+            self.lib.pdf_batch_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_iteration_kernel(self, k, min_pixels=0, level_kernel=True):
+        self._chk(self.lib.pdf_batch_set_iteration_kernel(self.ctx, k, min_pixels, int(level_kernel)))
+
+    def set_inputs(self, pair, I1, Z1, I2, Z2):
+        a = [np.ascontiguousarray(x, np.float32) for x in (I1, Z1, I2, Z2)]
+        self._chk(self.lib.pdf_batch_set_inputs(self.ctx, pair, *[_fptr(x) for x in a]))
+
+    def solve(self, graph=True):
+        self._chk(self.lib.pdf_batch_solve(self.ctx, 1 if graph else 0))
+
+    def synchronize(self):
+        self._chk(self.lib.pdf_batch_synchronize(self.ctx))
+
+    def flow(self, pair):
+        r = self.rows[pair]
+        c = r * 4 // 3
+        out = [np.empty(r * c, np.float32) for _ in range(3)]
+        self._chk(self.lib.pdf_batch_copy_flow(self.ctx, pair, *[_fptr(x) for x in out]))
+        return tuple(out)
+
+    def launch_count(self):
+        n = ctypes.c_uint()
+        self._chk(self.lib.pdf_batch_launch_count(self.ctx, ctypes.byref(n)))
+        return n.value

@@ -1,0 +1,59 @@
+"""Batched ragged solve: every pair of a mixed batch equals its single-pair solve."""
+import sys
+
+import numpy as np
+import pytest
+
+from pdflow import Batch, Solver, synth_pair_float
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def single(rows, seed, iters=None):
+    cam = 480 // rows
+    I1, Z1, I2, Z2 = synth_pair_float(seed, 480 // cam, 640 // cam)
+    g = Solver(rows, cam)
+    return (I1, Z1, I2, Z2), g.solve_pair(I1, Z1, I2, Z2, iters=iters)
+
+
+@pytest.mark.parametrize("k,graph", [(4, True), (2, False), (0, True)])
+def test_mixed_batch_matches_single_pairs(gpu_available, k, graph):
+    rows = [480, 240, 120, 60, 240, 30, 120, 15, 480]
+    say(f"mixed batch of {len(rows)} pairs, k={k}, graph={graph}")
+    b = Batch(rows)
+    if k:
+        b.set_iteration_kernel(k, 4800)
+    else:
+        b.set_iteration_kernel(0, 0, level_kernel=False)
+    want = []
+    for i, r in enumerate(rows):
+        inputs, flow = single(r, 500 + i)
+        b.set_inputs(i, *inputs)
+        want.append(flow)
+    for rep in range(2):
+        b.solve(graph=graph)
+        b.synchronize()
+        for i, r in enumerate(rows):
+            for a, w in zip(b.flow(i), want[i]):
+                np.testing.assert_array_equal(a, w, err_msg=f"pair {i} rows {r} rep {rep}")
+    say(f"  {b.launch_count()} launches per batched solve")
+
+
+def test_batch_with_custom_schedules(gpu_available):
+    """Per-pair iteration counts (as the learned warm start uses), odd counts included."""
+    rows = [240, 120, 240]
+    scheds = [[10, 11, 13, 17, 21], [3, 5, 7, 9], [40, 55, 70, 85, 100]]
+    say("batch with custom schedules")
+    b = Batch(rows, iters=scheds)
+    for i, r in enumerate(rows):
+        inputs, flow = single(r, 700 + i, iters=scheds[i])
+        b.set_inputs(i, *inputs)
+        b.solve()
+        b.synchronize()
+        for a, w in zip(b.flow(i), flow):
+            np.testing.assert_array_equal(a, w)
