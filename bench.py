@@ -60,6 +60,10 @@ def main():
     ap.add_argument("--tb-k", type=int, default=None,
                     help="iterations per launch of the temporally blocked kernel (1 = off)")
     ap.add_argument("--tb-min", type=int, default=None, help="smallest level (pixels) it runs on")
+    ap.add_argument("--workload", choices=("plain", "batched", "mixed"), default="plain",
+                    help="plain: one pair per step; batched: --batch-pairs pairs of --rows per "
+                         "step; mixed: 128 pairs (50%% 480x640, 30%% 240x320, 20%% 120x160)")
+    ap.add_argument("--batch-pairs", type=int, default=256)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -77,6 +81,8 @@ def main():
     if args.rows not in (15, 30, 60, 120, 240, 480):
         print("bench.py: --rows must be one of 15, 30, 60, 120, 240, 480", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "plain":
+        return run_batched(args)
     rows, cam_mode = args.rows, 480 // args.rows   # input frames at the solve resolution
     table = level_table(rows, cam_mode)
     iters = [t[4] for t in table]
@@ -166,6 +172,72 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(rows, cam_mode, args.seed, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def run_batched(args):
+    """Batched ragged solve: one step = one batched solve of all pairs (shared launches)."""
+    from pdflow import Batch, level_table, synth_pair_float
+    if args.workload == "mixed":
+        rows_list = [480] * 64 + [240] * 38 + [120] * 26
+        name = "batched plain PD-Flow, 128 mixed pairs (64x 480x640, 38x 240x320, 26x 120x160)"
+    else:
+        rows_list = [args.rows] * args.batch_pairs
+        name = f"batched plain PD-Flow, {args.batch_pairs} pairs of {args.rows}x{args.rows * 4 // 3}"
+    b = Batch(rows_list)
+    if args.tb_k is not None or args.tb_min is not None:
+        b.set_iteration_kernel(args.tb_k if args.tb_k is not None else 4,
+                               args.tb_min if args.tb_min is not None else 4800)
+    distinct = {}
+    for i, r in enumerate(rows_list):
+        key = (r, i % 16)
+        if key not in distinct:
+            cam = 480 // r
+            distinct[key] = synth_pair_float(args.seed + 1000 * r + i % 16, 480 // cam, 640 // cam)
+        b.set_inputs(i, *distinct[key])
+    log(f"bench: {name}")
+    for _ in range(max(1, args.warmup)):
+        b.solve(graph=not args.no_graph)
+    b.synchronize()
+    log(f"bench: {b.launch_count()} launches per batched solve")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.solve(graph=not args.no_graph)
+    b.synchronize()
+    el = time.perf_counter() - t0
+    pairs_s = len(rows_list) * args.steps / el
+    log(f"bench: {args.steps} batched solves in {el:.3f} s -> {pairs_s:.1f} pairs/s")
+    b.enable_kernel_timing(True)
+    b.solve(graph=False)
+    launches, kms, pix = b.kernel_timing()
+    b.enable_kernel_timing(False)
+    achieved = BYTES_PER_PIXEL_ITER * pix / (kms * 1e-3) / 1e9
+    log(f"bench: PD launches {launches}, {kms:.3f} ms, {achieved:.0f} GB/s algorithmic")
+    for i in (0, len(rows_list) - 1):
+        for a in b.flow(i):
+            assert np.isfinite(a).all()
+    pix_pair = {r: int(sum(rr * c * n for (_, rr, c, _, n) in level_table(r, 480 // r))) for r in set(rows_list)}
+    out = {
+        "metric": METRIC,
+        "value": round(pairs_s, 3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic (seeded RGB-D generator, 16 seeds per size from {args.seed})",
+        "config": {"workload": name + (", hipGraph" if not args.no_graph else ""),
+                   "pairs_per_step": len(rows_list),
+                   "pixel_iterations_per_step": int(sum(pix_pair[r] for r in rows_list))},
+        "roofline": {"bound": "hbm", "kernel": "kb_pd_tb / kb_pd_level (batched primal-dual iterations)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "launches": launches, "bytes_per_pixel_iteration": BYTES_PER_PIXEL_ITER},
+    }
     print(json.dumps(out), flush=True)
 
 

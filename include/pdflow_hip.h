@@ -194,6 +194,10 @@ int pdf_batch_flow_device(pdf_batch *b, unsigned pair, const float **dx, const f
 int pdf_batch_copy_flow(pdf_batch *b, unsigned pair, float *dx, float *dy, float *dz);
 /* number of kernel launches one batched solve makes */
 int pdf_batch_launch_count(const pdf_batch *b, unsigned *launches);
+/* HIP-event timing of the iteration launches, as pdf_enable_kernel_timing (solves run
+ * eagerly while enabled) */
+int pdf_batch_enable_kernel_timing(pdf_batch *b, int enable);
+int pdf_batch_kernel_timing(pdf_batch *b, uint64_t *launches, double *total_ms, uint64_t *pixel_iters);
 
 /* ---- device memory helpers for callers without a HIP binding ------------------------ */
 int pdf_device_alloc(void **ptr, size_t bytes);

@@ -57,6 +57,7 @@ struct Launch {
     int flag;
     size_t args_off, first_off;
     unsigned n, blocks;
+    uint64_t pixel_iters;   // PD launches: pixel-iterations the launch performs
 };
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -81,6 +82,12 @@ struct pdf_batch {
     char *dev_tables = nullptr;
     size_t dev_tables_bytes = 0;
     hipGraphExec_t graph = nullptr;
+    // HIP-event timing of the iteration launches (eager solves only)
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    std::vector<uint64_t> event_pix;
+    uint64_t timed_launches = 0, timed_pix = 0;
+    double timed_ms = 0.0;
     std::string err;
 };
 
@@ -212,7 +219,7 @@ void build_plan(pdf_batch *b)
                 blocks.push_back(pdf::batch_blocks_per_level(args.back().rows_i * args.back().cols_i));
             }
         }
-        Launch L{Launch::Setup, pdf::BatchKernel::Pyramid, 0, 0, 0, 0, 0, 0};
+        Launch L{Launch::Setup, pdf::BatchKernel::Pyramid, 0, 0, 0, 0, 0, 0, 0};
         add_launch(b, L, args, blocks);
     }
     for (unsigned s = 0; s < max_steps; s++) {
@@ -231,7 +238,7 @@ void build_plan(pdf_batch *b)
                 args.push_back(level_args(b, p));
                 blocks.push_back(pdf::batch_blocks_per_level(p.rows_i * p.cols_i));
             }
-            add_launch(b, Launch{Launch::Setup, k, 0, flag, 0, 0, 0, 0}, args, blocks);
+            add_launch(b, Launch{Launch::Setup, k, 0, flag, 0, 0, 0, 0, 0}, args, blocks);
         };
         setup(pdf::BatchKernel::LevelInit, s > 0 ? 1 : 0);
         setup(pdf::BatchKernel::RijGradients, 0);
@@ -253,7 +260,9 @@ void build_plan(pdf_batch *b)
                 who.push_back((unsigned)j);
             }
             if (args.empty()) return false;
-            add_launch(b, Launch{Launch::PD, pdf::BatchKernel::LevelInit, k, 0, 0, 0, 0, 0}, args, blocks);
+            uint64_t pix = 0;
+            for (size_t q = 0; q < args.size(); q++) pix += (uint64_t)args[q].rows * args[q].cols * args[q].iters;
+            add_launch(b, Launch{Launch::PD, pdf::BatchKernel::LevelInit, k, 0, 0, 0, 0, 0, pix}, args, blocks);
             for (unsigned j : who) {
                 rem[j] -= (unsigned)(k == 0 ? rem[j] : (k == 1 ? 1 : k));
                 swap_state(b->pairs[active[j]]);
@@ -290,15 +299,45 @@ int ensure_plan(pdf_batch *b)
     return PDF_OK;
 }
 
-int run_plan(pdf_batch *b)
+int run_plan(pdf_batch *b, bool timed)
 {
     for (const Launch &l : b->launches) {
         pdf::BatchTable t{b->dev_tables + l.args_off, (const unsigned *)(b->dev_tables + l.first_off), l.n};
+        const bool ev = timed && l.kind == Launch::PD;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (ev) {
+            BHIP(b, hipEventCreate(&e0));
+            BHIP(b, hipEventCreate(&e1));
+            BHIP(b, hipEventRecord(e0, b->stream));
+        }
         const hipError_t e = l.kind == Launch::Setup
                                  ? pdf::launch_batch(l.setup, t, l.blocks, l.flag, b->stream)
                                  : pdf::launch_pd_batch(l.k, t, l.blocks, b->stream);
         if (e != hipSuccess) return bfail(b, PDF_ERR_HIP, std::string("batched launch: ") + hipGetErrorString(e));
+        if (ev) {
+            BHIP(b, hipEventRecord(e1, b->stream));
+            b->events.emplace_back(e0, e1);
+            b->event_pix.push_back(l.pixel_iters);
+        }
     }
+    return PDF_OK;
+}
+
+int finish_timing(pdf_batch *b)
+{
+    if (b->events.empty()) return PDF_OK;
+    BHIP(b, hipEventSynchronize(b->events.back().second));
+    for (size_t i = 0; i < b->events.size(); i++) {
+        float ms = 0.f;
+        BHIP(b, hipEventElapsedTime(&ms, b->events[i].first, b->events[i].second));
+        b->timed_ms += ms;
+        b->timed_pix += b->event_pix[i];
+        b->timed_launches++;
+        (void)hipEventDestroy(b->events[i].first);
+        (void)hipEventDestroy(b->events[i].second);
+    }
+    b->events.clear();
+    b->event_pix.clear();
     return PDF_OK;
 }
 
@@ -406,6 +445,7 @@ int pdf_batch_destroy(pdf_batch *b)
 {
     if (!b) return PDF_ERR_ARG;
     (void)hipStreamSynchronize(b->stream);
+    for (auto &ev : b->events) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     if (b->dev_tables) (void)hipFree(b->dev_tables);
     if (b->own_stream) (void)hipStreamDestroy(b->stream);
@@ -461,10 +501,10 @@ int pdf_batch_solve(pdf_batch *b, int use_graph)
     if (!b) return PDF_ERR_ARG;
     int rc = ensure_plan(b);
     if (rc) return rc;
-    if (!use_graph) return run_plan(b);
+    if (!use_graph || b->timing) return run_plan(b, b->timing);
     if (!b->graph) {
         BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        rc = run_plan(b);
+        rc = run_plan(b, false);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(b->stream, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -481,6 +521,29 @@ int pdf_batch_synchronize(pdf_batch *b)
 {
     if (!b) return PDF_ERR_ARG;
     BHIP(b, hipStreamSynchronize(b->stream));
+    return finish_timing(b);
+}
+
+int pdf_batch_enable_kernel_timing(pdf_batch *b, int enable)
+{
+    if (!b) return PDF_ERR_ARG;
+    int rc = pdf_batch_synchronize(b);
+    if (rc) return rc;
+    b->timing = enable != 0;
+    b->timed_launches = 0;
+    b->timed_pix = 0;
+    b->timed_ms = 0.0;
+    return PDF_OK;
+}
+
+int pdf_batch_kernel_timing(pdf_batch *b, uint64_t *launches, double *total_ms, uint64_t *pixel_iters)
+{
+    if (!b) return PDF_ERR_ARG;
+    int rc = pdf_batch_synchronize(b);
+    if (rc) return rc;
+    if (launches) *launches = b->timed_launches;
+    if (total_ms) *total_ms = b->timed_ms;
+    if (pixel_iters) *pixel_iters = b->timed_pix;
     return PDF_OK;
 }
 

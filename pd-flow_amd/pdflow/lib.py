@@ -114,6 +114,9 @@ def hip_library():
         "pdf_batch_flow_device": ([vp, c_uint, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), up_, up_], c_int),
         "pdf_batch_copy_flow": ([vp, c_uint, fp, fp, fp], c_int),
         "pdf_batch_launch_count": ([vp, up_], c_int),
+        "pdf_batch_enable_kernel_timing": ([vp, c_int], c_int),
+        "pdf_batch_kernel_timing": ([vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_uint64)], c_int),
     })
     for stage in ("assign_zeros", "upsample", "rij", "image_gradients", "warping",
                   "mu_and_step_sizes", "gradient", "dual_variables", "divergence",
@@ -366,6 +369,19 @@ class Batch:
         out = [np.empty(r * c, np.float32) for _ in range(3)]
         self._chk(self.lib.pdf_batch_copy_flow(self.ctx, pair, *[_fptr(x) for x in out]))
         return tuple(out)
+
+    def enable_kernel_timing(self, on=True):
+        self._chk(self.lib.pdf_batch_enable_kernel_timing(self.ctx, 1 if on else 0))
+
+    def kernel_timing(self):
+        n, ms, pix = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_uint64()
+        self._chk(self.lib.pdf_batch_kernel_timing(self.ctx, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(pix)))
+        return n.value, ms.value, pix.value
+
+    def inputs_device(self, pair):
+        ptrs = [ctypes.POINTER(ctypes.c_float)() for _ in range(4)]
+        self._chk(self.lib.pdf_batch_inputs_device(self.ctx, pair, *[ctypes.byref(x) for x in ptrs]))
+        return [ctypes.cast(x, ctypes.c_void_p).value for x in ptrs]
 
     def launch_count(self):
         n = ctypes.c_uint()
