@@ -132,6 +132,15 @@ int pdf_motion_field_device(pdf_ctx *ctx, const float **dx, const float **dy,
  * level, 15 fewer per coarser level).  The finest level's flow is left in dx/dy/dz. */
 int pdf_solve(pdf_ctx *ctx, const unsigned *num_max_iter);
 
+/* The solve started at level start_level (coarse-first index) from a warm start instead of the
+ * upsampled coarser solution (no reference counterpart; the step before the solve of
+ * scene_flow_impair.cpp:95-168).  warm_device holds three consecutive device planes of that
+ * level's size, column-major: du and dv in pixels of that level, dw in metres.  The level
+ * starts with du_prev, dv_prev = du, dv, dw_new = dw_acc = dw and all duals at 0; finer levels
+ * proceed as in pdf_solve.  Coarser levels are skipped. */
+int pdf_solve_warm(pdf_ctx *ctx, const unsigned *num_max_iter, unsigned start_level,
+                   const float *warm_device);
+
 /* Same as pdf_solve, but replays a HIP graph captured on first use (one graph per
  * schedule and old/new parity). */
 int pdf_solve_graph(pdf_ctx *ctx, const unsigned *num_max_iter);
@@ -181,6 +190,13 @@ int pdf_batch_set_iteration_kernel(pdf_batch *b, int k, unsigned min_pixels, int
 /* device pointers of pair's four input planes (the caller may fill them directly) */
 int pdf_batch_inputs_device(pdf_batch *b, unsigned pair, float **I1, float **Z1, float **I2,
                             float **Z2);
+/* Warm start of one pair (as pdf_solve_warm): the pair's solve starts at start_level (0 =
+ * normal coarse-to-fine) from the three planes (du, dv, dw at that level, column-major)
+ * returned in *warm_device, which the caller fills before pdf_batch_solve.  Changing the
+ * start level invalidates the recorded plan. */
+int pdf_batch_set_start_level(pdf_batch *b, unsigned pair, unsigned start_level, float **warm_device);
+/* per-pair iterations per level, coarse first (invalidates the plan when they change) */
+int pdf_batch_set_schedule(pdf_batch *b, unsigned pair, const unsigned *num_max_iter);
 /* blocking upload of one pair's frames from host planes */
 int pdf_batch_set_inputs(pdf_batch *b, unsigned pair, const float *I1, const float *Z1,
                          const float *I2, const float *Z2);

@@ -43,6 +43,8 @@ struct Pair {
     unsigned rows, cols, cam_mode, width, height, ctf_levels, pyr_levels, lw;
     unsigned iters[PDF_MAX_CTF_LEVELS];
     float *input[4];                                       // I1, Z1, I2, Z2
+    unsigned start_level;                                  // 0, or a warm-started level
+    float *warm;                                           // du, dv, dw planes of start_level
     float *pyr[PDF_NUM_PYR_KINDS][PDF_MAX_PYR_LEVELS];
     float *f[PDF_NUM_FIELDS];
     float *alt[PDF_NUM_FIELDS];
@@ -133,6 +135,7 @@ pdf::LevelArgs level_args(const pdf_batch *b, const Pair &p)
     a.lambda_d = b->lambda_d;
     a.mu = b->mu;
     a.inv_f = 2.f * b->tan_half / (float)p.cols;   // finest cols (.cu:1152)
+    a.warm = p.warm;
     std::memcpy(a.g, b->g_mask, sizeof(a.g));
     return a;
 }
@@ -223,24 +226,27 @@ void build_plan(pdf_batch *b)
         add_launch(b, L, args, blocks);
     }
     for (unsigned s = 0; s < max_steps; s++) {
-        std::vector<unsigned> active;
+        std::vector<unsigned> active, warm_now, cold_now;
         for (unsigned i = 0; i < b->pairs.size(); i++) {
             Pair &p = b->pairs[i];
-            if (s >= p.ctf_levels) continue;
+            if (s >= p.ctf_levels || s < p.start_level) continue;
             level_dims(p, s, &p.rows_i, &p.cols_i, &p.level_image);
             active.push_back(i);
+            (p.start_level > 0 && s == p.start_level ? warm_now : cold_now).push_back(i);
         }
-        auto setup = [&](pdf::BatchKernel k, int flag) {
+        auto setup_for = [&](pdf::BatchKernel k, int flag, const std::vector<unsigned> &who) {
             std::vector<pdf::LevelArgs> args;
             std::vector<unsigned> blocks;
-            for (unsigned i : active) {
+            for (unsigned i : who) {
                 const Pair &p = b->pairs[i];
                 args.push_back(level_args(b, p));
                 blocks.push_back(pdf::batch_blocks_per_level(p.rows_i * p.cols_i));
             }
             add_launch(b, Launch{Launch::Setup, k, 0, flag, 0, 0, 0, 0, 0}, args, blocks);
         };
-        setup(pdf::BatchKernel::LevelInit, s > 0 ? 1 : 0);
+        auto setup = [&](pdf::BatchKernel k, int flag) { setup_for(k, flag, active); };
+        setup_for(pdf::BatchKernel::LevelInit, s > 0 ? 1 : 0, cold_now);
+        setup_for(pdf::BatchKernel::WarmInit, 0, warm_now);
         setup(pdf::BatchKernel::RijGradients, 0);
         setup(pdf::BatchKernel::WarpMu, 0);
 
@@ -409,6 +415,7 @@ int pdf_batch_create(pdf_batch **out, unsigned npairs, const unsigned *rows, con
         bytes += 4 * rnd(P0);
         for (unsigned l = 0; l < p.pyr_levels; l++) bytes += PDF_NUM_PYR_KINDS * rnd(P0 >> (2 * l));
         bytes += (sizeof(kUsed) / sizeof(kUsed[0]) + pdf::S_NUM) * rnd(P0);
+        bytes += 3 * rnd(P0);                                   // warm-start planes
     }
     hipError_t e = hipMalloc((void **)&b->arena, bytes);
     if (e != hipSuccess) {
@@ -426,6 +433,8 @@ int pdf_batch_create(pdf_batch **out, unsigned npairs, const unsigned *rows, con
             for (int k = 0; k < PDF_NUM_PYR_KINDS; k++) p.pyr[k][l] = take(P0 >> (2 * l));
         for (int id : kUsed) p.f[id] = take(P0);
         for (int k = 0; k < pdf::S_NUM; k++) p.alt[kState[k]] = take(P0);
+        p.warm = take(3 * P0);
+        p.start_level = 0;
     }
     e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemsetAsync(b->arena, 0, bytes, b->stream);
@@ -469,6 +478,30 @@ int pdf_batch_set_iteration_kernel(pdf_batch *b, int k, unsigned min_pixels, int
     b->tb_k = k > 1 ? k : 0;
     b->tb_min_pixels = min_pixels;
     b->level_kernel = level_kernel != 0;
+    return PDF_OK;
+}
+
+int pdf_batch_set_start_level(pdf_batch *b, unsigned pair, unsigned start_level, float **warm_device)
+{
+    if (!b || pair >= b->pairs.size()) return bfail(b, PDF_ERR_ARG, "bad pair index");
+    Pair &p = b->pairs[pair];
+    if (start_level >= p.ctf_levels) return bfail(b, PDF_ERR_ARG, "start level beyond the finest level");
+    if (p.start_level != start_level) {
+        drop_plan(b);
+        p.start_level = start_level;
+    }
+    if (warm_device) *warm_device = p.warm;
+    return PDF_OK;
+}
+
+int pdf_batch_set_schedule(pdf_batch *b, unsigned pair, const unsigned *num_max_iter)
+{
+    if (!b || pair >= b->pairs.size() || !num_max_iter) return bfail(b, PDF_ERR_ARG, "bad pair index or schedule");
+    Pair &p = b->pairs[pair];
+    if (std::memcmp(p.iters, num_max_iter, p.ctf_levels * sizeof(unsigned)) != 0) {
+        drop_plan(b);
+        std::memcpy(p.iters, num_max_iter, p.ctf_levels * sizeof(unsigned));
+    }
     return PDF_OK;
 }
 

@@ -130,6 +130,7 @@ pdf::LevelArgs level_args(const pdf_ctx *c)
     a.lambda_d = c->lambda_d;
     a.mu = c->mu;
     a.inv_f = 2.f * c->tan_half / (float)c->cols;  // finest cols at every level (.cu:1152)
+    a.warm = nullptr;
     std::memcpy(a.g, c->g_mask, sizeof(a.g));
     return a;
 }
@@ -552,11 +553,13 @@ int pdf_motion_field_device(pdf_ctx *c, const float **dx, const float **dy, cons
     return PDF_OK;
 }
 
-static int fused_level(pdf_ctx *c, unsigned i, const unsigned *iters)
+static int fused_level(pdf_ctx *c, unsigned i, const unsigned *iters, const float *warm = nullptr)
 {
     // the same stages as the bridges below, in four set-up launches (pdflow_setup.hip)
-    const pdf::LevelArgs a = level_args(c);
-    int rc = check_launch(c, pdf::launch_level_init(a, i > 0, c->stream), "level init");
+    pdf::LevelArgs a = level_args(c);
+    a.warm = warm;
+    int rc = warm ? check_launch(c, pdf::launch_warm_init(a, c->stream), "warm start")
+                  : check_launch(c, pdf::launch_level_init(a, i > 0, c->stream), "level init");
     if (!rc) rc = check_launch(c, pdf::launch_rij_gradients(a, c->stream), "rij + gradients");
     if (!rc) rc = check_launch(c, pdf::launch_warp_mu(a, false, c->stream), "warp + mu");
     if (!rc) rc = pdf_pd_iterations(c, iters[i]);
@@ -605,6 +608,22 @@ int pdf_solve(pdf_ctx *c, const unsigned *num_max_iter)
     unsigned it[6];
     resolve_schedule(c, num_max_iter, it);
     return solve_sequence(c, it);
+}
+
+int pdf_solve_warm(pdf_ctx *c, const unsigned *num_max_iter, unsigned start_level, const float *warm_device)
+{
+    if (!c || !warm_device) return fail(c, PDF_ERR_ARG, "null warm-start planes");
+    if (start_level >= c->ctf_levels) return fail(c, PDF_ERR_ARG, "start level beyond the finest level");
+    unsigned it[6];
+    resolve_schedule(c, num_max_iter, it);
+    for (unsigned i = start_level; i < c->ctf_levels; i++) {
+        unsigned rows_i, cols_i, level_image;
+        level_dims_for(c, i, &rows_i, &cols_i, &level_image);
+        int rc = pdf_allocate_level(c, rows_i, cols_i, i, level_image);
+        if (!rc) rc = fused_level(c, i, it, i == start_level ? warm_device : nullptr);
+        if (rc) return rc;
+    }
+    return PDF_OK;
 }
 
 int pdf_solve_graph(pdf_ctx *c, const unsigned *num_max_iter)

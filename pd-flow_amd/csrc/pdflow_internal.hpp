@@ -16,6 +16,7 @@ struct LevelArgs {
     unsigned rows, cols;                     // rows_i, cols_i
     float lambda_i, lambda_d, mu;
     float inv_f;                             // 2 tan(fovh/2) / finest cols (motion field)
+    const float *warm;                       // warm start: du, dv, dw planes of this level
     float g[25];                             // Gaussian mask as given to readParameters
 };
 
@@ -77,7 +78,7 @@ __device__ __forceinline__ BatchItem batch_item(const BatchTable &t)
     return {lo, b - t.first[lo]};
 }
 
-enum class BatchKernel { LevelInit, RijGradients, WarpMu, FilterMotion, Pyramid };
+enum class BatchKernel { LevelInit, RijGradients, WarpMu, FilterMotion, Pyramid, WarmInit };
 
 // Iterations per launch supported by the temporally blocked kernel.
 constexpr int kTBMaxK = 8;
@@ -104,6 +105,7 @@ hipError_t launch_filter(const LevelArgs &a, hipStream_t s);
 // fused set-up of the solve path (pdflow_setup.hip)
 hipError_t launch_level_init(const LevelArgs &a, bool has_coarse, hipStream_t s);
 hipError_t launch_rij_gradients(const LevelArgs &a, hipStream_t s);
+hipError_t launch_warm_init(const LevelArgs &a, hipStream_t s);
 hipError_t launch_warp_mu(const LevelArgs &a, bool write_steps, hipStream_t s);
 hipError_t launch_filter_motion(const LevelArgs &a, hipStream_t s);
 hipError_t launch_pyramid2(const Pyr2Args &a, hipStream_t s);

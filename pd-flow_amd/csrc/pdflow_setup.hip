@@ -87,6 +87,26 @@ __device__ __forceinline__ void level_init_body(const LevelArgs &a, unsigned blo
     a.f[PDF_DW_ACC][index] = acc[2];
 }
 
+// ------------------------------------------------------------------ warm start
+// Replaces K2-K4 at the level a solve starts from: du_prev, dv_prev (pixels at this level)
+// and dw (metres) come from three given planes, every dual and increment starts at 0
+// (the injection point of the learned initialisation; no reference counterpart).
+__device__ __forceinline__ void warm_init_body(const LevelArgs &a, unsigned block)
+{
+    const unsigned index = block * kBlock + threadIdx.x;
+    const unsigned P = a.rows * a.cols;
+    if (index >= P) return;
+    const float du = a.warm[index], dv = a.warm[P + index], dw = a.warm[2 * P + index];
+#pragma unroll
+    for (int k = 0; k < 4; k++) a.f[kInitZero[k]][index] = 0.f;
+    a.f[PDF_DU_PREV][index] = du;
+    a.f[PDF_DV_PREV][index] = dv;
+    a.f[PDF_DW_NEW][index] = dw;
+    a.f[PDF_DW_ACC][index] = dw;
+#pragma unroll
+    for (int q = 3; q < 10; q++) a.f[kUpDst[q]][index] = 0.f;   // pd and the six TV duals
+}
+
 // ------------------------------------------------------------------ K5 + K6
 __device__ __forceinline__ float rweight(float d1, float d2)
 {
@@ -338,6 +358,7 @@ __device__ __forceinline__ void pyramid_body(const PyrArgs &p, unsigned block)
 // its pair's arguments in a device table (see BatchTable).
 __global__ void __launch_bounds__(kBlock) k_level_init(LevelArgs a, int has_coarse) { level_init_body(a, blockIdx.x, has_coarse); }
 __global__ void __launch_bounds__(kBlock) k_rij_gradients(LevelArgs a) { rij_gradients_body(a, blockIdx.x); }
+__global__ void __launch_bounds__(kBlock) k_warm_init(LevelArgs a) { warm_init_body(a, blockIdx.x); }
 __global__ void __launch_bounds__(kBlock) k_warp_mu(LevelArgs a, int write_steps) { warp_mu_body(a, blockIdx.x, write_steps); }
 __global__ void __launch_bounds__(kBlock) k_filter_motion(LevelArgs a) { filter_motion_body(a, blockIdx.x); }
 __global__ void __launch_bounds__(kBlock) k_pyramid2(Pyr2Args a) { pyramid_body(a.frame[blockIdx.y], blockIdx.x); }
@@ -346,6 +367,11 @@ __global__ void __launch_bounds__(kBlock) kb_level_init(BatchTable t, int has_co
 {
     const BatchItem it = batch_item(t);
     level_init_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, has_coarse);
+}
+__global__ void __launch_bounds__(kBlock) kb_warm_init(BatchTable t)
+{
+    const BatchItem it = batch_item(t);
+    warm_init_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block);
 }
 __global__ void __launch_bounds__(kBlock) kb_rij_gradients(BatchTable t)
 {
@@ -375,6 +401,14 @@ hipError_t launch_level_init(const LevelArgs &a, bool has_coarse, hipStream_t s)
     const unsigned n = a.rows * a.cols;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_level_init, grid_for(n), dim3(kBlock), 0, s, a, has_coarse ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_warm_init(const LevelArgs &a, hipStream_t s)
+{
+    const unsigned n = a.rows * a.cols;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_warm_init, grid_for(n), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 
@@ -425,6 +459,7 @@ hipError_t launch_batch(BatchKernel k, const BatchTable &t, unsigned total_block
     case BatchKernel::WarpMu: hipLaunchKernelGGL(kb_warp_mu, grid, block, 0, s, t); break;
     case BatchKernel::FilterMotion: hipLaunchKernelGGL(kb_filter_motion, grid, block, 0, s, t); break;
     case BatchKernel::Pyramid: hipLaunchKernelGGL(kb_pyramid, grid, block, 0, s, t); break;
+    case BatchKernel::WarmInit: hipLaunchKernelGGL(kb_warm_init, grid, block, 0, s, t); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

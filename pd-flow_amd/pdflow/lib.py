@@ -114,6 +114,9 @@ def hip_library():
         "pdf_batch_flow_device": ([vp, c_uint, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), up_, up_], c_int),
         "pdf_batch_copy_flow": ([vp, c_uint, fp, fp, fp], c_int),
         "pdf_batch_launch_count": ([vp, up_], c_int),
+        "pdf_batch_set_start_level": ([vp, c_uint, c_uint, fpp], c_int),
+        "pdf_batch_set_schedule": ([vp, c_uint, up_], c_int),
+        "pdf_solve_warm": ([vp, up, c_uint, vp], c_int),
         "pdf_batch_enable_kernel_timing": ([vp, c_int], c_int),
         "pdf_batch_kernel_timing": ([vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_uint64)], c_int),
@@ -286,6 +289,11 @@ class Solver:
         self._chk(self.lib.pdf_copy_motion_field(self.ctx, _fptr(dx), _fptr(dy), _fptr(dz)))
         return dx, dy, dz
 
+    def solve_warm(self, start_level, warm_device_ptr, iters=None):
+        """Solve from `start_level` initialised by the 3 device planes at warm_device_ptr."""
+        arr = None if iters is None else (ctypes.c_uint * len(iters))(*iters)
+        self._chk(self.lib.pdf_solve_warm(self.ctx, arr, start_level, ctypes.c_void_p(warm_device_ptr)))
+
     def solve_pair_device(self, bufs, iters=None, graph=False):
         """One pair from four device buffers (I1, Z1, I2, Z2); flow stays on the device."""
         arr = None if iters is None else (ctypes.c_uint * len(iters))(*iters)
@@ -382,6 +390,16 @@ class Batch:
         ptrs = [ctypes.POINTER(ctypes.c_float)() for _ in range(4)]
         self._chk(self.lib.pdf_batch_inputs_device(self.ctx, pair, *[ctypes.byref(x) for x in ptrs]))
         return [ctypes.cast(x, ctypes.c_void_p).value for x in ptrs]
+
+    def set_start_level(self, pair, level):
+        """Warm-start pair at `level`; returns the device address of its du, dv, dw planes."""
+        ptr = ctypes.POINTER(ctypes.c_float)()
+        self._chk(self.lib.pdf_batch_set_start_level(self.ctx, pair, level, ctypes.byref(ptr)))
+        return ctypes.cast(ptr, ctypes.c_void_p).value
+
+    def set_schedule(self, pair, iters):
+        arr = (ctypes.c_uint * 6)(*(list(iters) + [0] * (6 - len(iters))))
+        self._chk(self.lib.pdf_batch_set_schedule(self.ctx, pair, arr))
 
     def launch_count(self):
         n = ctypes.c_uint()
