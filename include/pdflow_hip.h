@@ -150,6 +150,13 @@ int pdf_solve_graph(pdf_ctx *ctx, const unsigned *num_max_iter);
 int pdf_solve_pair_device(pdf_ctx *ctx, const float *dI1, const float *dZ1, const float *dI2,
                           const float *dZ2, const unsigned *num_max_iter, int use_graph);
 
+/* Streaming: the live viewer's loop (main_scene_flow_visualization.cpp:114-130,
+ * scene_flow_visualization.cpp:112-208): ingest one new frame from device planes, build its
+ * pyramid (the previous frame's becomes "old"), and, from the second frame on, solve for the
+ * flow from the previous frame to this one.  *solved (optional) tells whether a solve ran. */
+int pdf_stream_frame_device(pdf_ctx *ctx, const float *dI, const float *dZ,
+                            const unsigned *num_max_iter, int use_graph, int *solved);
+
 /* ---- inspection (no reference counterpart: test and tooling access) ----------------- */
 
 /* rows_i*cols_i values of one plane of the current level (blocking) */

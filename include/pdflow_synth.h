@@ -41,6 +41,12 @@ int pdf_synth_pair_float(uint64_t seed, unsigned rows, unsigned cols,
 int pdf_synth_static_float(uint64_t seed, unsigned rows, unsigned cols,
                            float *I, float *Z);
 
+/* Frame t of a moving sequence of the same scene: every patch's motion applied t times
+ * (t = 0 and t = 1 render the scene of pdf_synth_pair without/with its frame-2 motion; frames
+ * t >= 1 carry intensity noise).  Used for the streaming mode, one new frame per solve. */
+int pdf_synth_sequence_float(uint64_t seed, unsigned rows, unsigned cols, unsigned t, float *I,
+                             float *Z);
+
 /* FNV-1a 64-bit hash of a byte buffer (used to pin generator outputs). */
 uint64_t pdf_synth_hash(const void *data, uint64_t nbytes);
 

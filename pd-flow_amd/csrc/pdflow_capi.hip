@@ -60,6 +60,7 @@ struct pdf_ctx {
     unsigned rows_i = 0, cols_i = 0, level = 0, level_image = 0;
     bool level_set = false;
     int parity = 0;  // number of old/new swaps mod 2 (selects graph variants)
+    unsigned frames_seen = 0;  // frames ingested by pdf_stream_frame_device
 
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -702,6 +703,21 @@ int pdf_solve_pair_device(pdf_ctx *c, const float *dI1, const float *dZ1, const 
         if (rc) return rc;
     }
     return use_graph ? pdf_solve_graph(c, num_max_iter) : pdf_solve(c, num_max_iter);
+}
+
+int pdf_stream_frame_device(pdf_ctx *c, const float *dI, const float *dZ, const unsigned *num_max_iter,
+                            int use_graph, int *solved)
+{
+    // the live loop of main_scene_flow_visualization.cpp:114-130: one pyramid per new frame,
+    // the previous frame's pyramid becomes "old" (.cu:203-221), then the solve
+    if (solved) *solved = 0;
+    int rc = pdf_copy_new_frames_device(c, dI, dZ);
+    if (!rc) rc = pdf_gaussian_pyramid(c, c->pyr_levels, c->cam_mode);
+    if (rc) return rc;
+    if (++c->frames_seen < 2) return PDF_OK;
+    rc = use_graph ? pdf_solve_graph(c, num_max_iter) : pdf_solve(c, num_max_iter);
+    if (!rc && solved) *solved = 1;
+    return rc;
 }
 
 int pdf_set_solve_options(pdf_ctx *c, int fused_setup, int level_kernel)

@@ -136,15 +136,18 @@ static void make_scene(uint64_t seed, unsigned rows, unsigned cols, patch_t *pat
     }
 }
 
-static patch_t moved_patch(const patch_t *p, const motion_t *m)
+static patch_t moved_patch_t(const patch_t *p, const motion_t *m, double t)
 {
+    /* the motion applied t times: rotation t*angle about the same axis, translation t*trans */
     patch_t q = *p;
-    q.p0 = vadd(p->p0, m->trans);
-    q.n = vrot(p->n, m->axis, m->angle);
-    q.e1 = vrot(p->e1, m->axis, m->angle);
-    q.e2 = vrot(p->e2, m->axis, m->angle);
+    q.p0 = vadd(p->p0, vscale(m->trans, t));
+    q.n = vrot(p->n, m->axis, m->angle * t);
+    q.e1 = vrot(p->e1, m->axis, m->angle * t);
+    q.e2 = vrot(p->e2, m->axis, m->angle * t);
     return q;
 }
+
+static patch_t moved_patch(const patch_t *p, const motion_t *m) { return moved_patch_t(p, m, 1.0); }
 
 /* ray-casts the patches for one pixel; returns the hit patch or -1, depth in *z,
  * local coordinates in *s, *t */
@@ -266,6 +269,29 @@ int pdf_synth_pair(uint64_t seed, unsigned rows, unsigned cols, uint8_t *i1, uin
 }
 
 static float depth_to_float(uint16_t z) { return (float)((double)z / 5000.0); }
+
+int pdf_synth_sequence_float(uint64_t seed, unsigned rows, unsigned cols, unsigned t, float *I,
+                             float *Z)
+{
+    if (rows == 0 || cols == 0) return 1;
+    patch_t ps[MAX_PATCHES], pt[MAX_PATCHES];
+    motion_t mot[MAX_PATCHES];
+    int n;
+    make_scene(seed, rows, cols, ps, mot, &n);
+    for (int i = 0; i < n; i++) pt[i] = moved_patch_t(&ps[i], &mot[i], (double)t);
+    const size_t P = (size_t)rows * cols;
+    uint8_t *a = (uint8_t *)malloc(P);
+    uint16_t *b = (uint16_t *)malloc(P * sizeof(uint16_t));
+    if (!a || !b) { free(a); free(b); return 2; }
+    render(seed, t, pt, n, rows, cols, t > 0, a, b, NULL, NULL);
+    for (size_t i = 0; i < P; i++) {
+        if (I) I[i] = (float)a[i];
+        if (Z) Z[i] = depth_to_float(b[i]);
+    }
+    free(a);
+    free(b);
+    return 0;
+}
 
 int pdf_synth_pair_float(uint64_t seed, unsigned rows, unsigned cols, float *I1, float *Z1,
                          float *I2, float *Z2, float *gt)

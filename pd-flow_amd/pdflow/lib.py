@@ -117,6 +117,7 @@ def hip_library():
         "pdf_batch_set_start_level": ([vp, c_uint, c_uint, fpp], c_int),
         "pdf_batch_set_schedule": ([vp, c_uint, up_], c_int),
         "pdf_solve_warm": ([vp, up, c_uint, vp], c_int),
+        "pdf_stream_frame_device": ([vp, vp, vp, up, c_int, ctypes.POINTER(c_int)], c_int),
         "pdf_batch_enable_kernel_timing": ([vp, c_int], c_int),
         "pdf_batch_kernel_timing": ([vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_uint64)], c_int),
@@ -289,6 +290,14 @@ class Solver:
         self._chk(self.lib.pdf_copy_motion_field(self.ctx, _fptr(dx), _fptr(dy), _fptr(dz)))
         return dx, dy, dz
 
+    def stream_frame_device(self, dI, dZ, iters=None, graph=True):
+        """Streaming: one new frame (device buffers); returns True when a solve ran."""
+        arr = None if iters is None else (ctypes.c_uint * len(iters))(*iters)
+        solved = ctypes.c_int(0)
+        self._chk(self.lib.pdf_stream_frame_device(self.ctx, dI.ptr, dZ.ptr, arr, 1 if graph else 0,
+                                                   ctypes.byref(solved)))
+        return bool(solved.value)
+
     def solve_warm(self, start_level, warm_device_ptr, iters=None):
         """Solve from `start_level` initialised by the 3 device planes at warm_device_ptr."""
         arr = None if iters is None else (ctypes.c_uint * len(iters))(*iters)
@@ -299,6 +308,15 @@ class Solver:
         arr = None if iters is None else (ctypes.c_uint * len(iters))(*iters)
         self._chk(self.lib.pdf_solve_pair_device(self.ctx, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr,
                                                  bufs[3].ptr, arr, 1 if graph else 0))
+
+    def copy_all_solutions(self):
+        """copyAllSolutions (pdflow_cudalib.cu:224-237): the level's flow and both frames'
+        depth, colour, xx, yy planes, as a dict of host arrays."""
+        r, c = self.level_dims()
+        names = ("dx", "dy", "dz", "depth", "depth_old", "colour", "colour_old", "xx", "xx_old", "yy", "yy_old")
+        out = {n: np.empty(r * c, np.float32) for n in names}
+        self._chk(self.lib.pdf_copy_all_solutions(self.ctx, *[_fptr(out[n]) for n in names]))
+        return out
 
     def solve_pair(self, I1, Z1, I2, Z2, iters=None, graph=False):
         """The pair evaluator end to end from host planes: returns (dx, dy, dz)."""

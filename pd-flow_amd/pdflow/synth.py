@@ -21,6 +21,7 @@ def _load():
         lib.pdf_synth_pair.argtypes = [ctypes.c_uint64, ctypes.c_uint, ctypes.c_uint, u8, u16, u8, u16, fp]
         lib.pdf_synth_pair_float.argtypes = [ctypes.c_uint64, ctypes.c_uint, ctypes.c_uint, fp, fp, fp, fp, fp]
         lib.pdf_synth_static_float.argtypes = [ctypes.c_uint64, ctypes.c_uint, ctypes.c_uint, fp, fp]
+        lib.pdf_synth_sequence_float.argtypes = [ctypes.c_uint64, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, fp, fp]
         lib.pdf_synth_hash.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
         lib.pdf_synth_hash.restype = ctypes.c_uint64
         _lib = lib
@@ -71,6 +72,17 @@ def synth_static_float(seed, rows, cols):
     rc = _load().pdf_synth_static_float(seed, rows, cols, _p(I, ctypes.c_float), _p(Z, ctypes.c_float))
     if rc:
         raise RuntimeError(f"pdf_synth_static_float failed ({rc})")
+    return I, Z
+
+
+def synth_sequence_float(seed, rows, cols, t):
+    """Frame t of the seeded moving scene (I, Z float planes)."""
+    P = rows * cols
+    I = np.zeros(P, np.float32)
+    Z = np.zeros(P, np.float32)
+    rc = _load().pdf_synth_sequence_float(seed, rows, cols, t, _p(I, ctypes.c_float), _p(Z, ctypes.c_float))
+    if rc:
+        raise RuntimeError(f"pdf_synth_sequence_float failed ({rc})")
     return I, Z
 
 

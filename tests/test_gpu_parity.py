@@ -265,3 +265,50 @@ def test_device_pair_path_matches_host_path(gpu_available):
         g.solve_pair_device(bufs, graph=graph)
         for a, b in zip(g.motion_field(), want):
             np.testing.assert_array_equal(a, b)
+
+
+def test_streaming_matches_oracle_pairs(gpu_available):
+    """Streaming (one pyramid per new frame, old/new swap) gives, for every new frame, the
+    flow the oracle computes for (previous frame, this frame)."""
+    say("streaming 240x320, 4 frames")
+    from pdflow import synth_sequence_float
+    from pdflow.lib import DeviceBuffer
+    g = Solver(240, 2)
+    o = Oracle(240, 2, threads=8)
+    prev = None
+    for t in range(4):
+        I, Z = synth_sequence_float(1234, 240, 320, t)
+        bI, bZ = DeviceBuffer(I.nbytes), DeviceBuffer(Z.nbytes)
+        bI.upload(I)
+        bZ.upload(Z)
+        solved = g.stream_frame_device(bI, bZ, graph=(t % 2 == 1))
+        o.copy_new_frames(I, Z)
+        o.gaussian_pyramid()
+        assert solved == (t > 0)
+        if t > 0:
+            o.solve()
+            for a, b, n in zip(g.motion_field(), o.motion_field(), ("dx", "dy", "dz")):
+                assert_same(a, b, f"frame {t} {n}")
+        prev = (bI, bZ)
+
+
+def test_copy_all_solutions_per_level(gpu_available):
+    """copyAllSolutions after each level (the live viewer's per-level export,
+    scene_flow_visualization.cpp:197-199), including the coarse levels' dx, which use the
+    finest cols in inv_f (.cu:1152)."""
+    say("copyAllSolutions per level")
+    g, o, _ = both(120, 4, seed=77)
+    for (i, r, c, limg, iters) in level_table(120, 4):
+        for s in (g, o):
+            s.allocate_level(r, c, i, limg)
+            for st in ["assign_zeros"] + (["upsample"] if i > 0 else []) + SETUP_STAGES:
+                s.stage(st)
+            s.pd_iterations(iters)
+            s.stage("filter")
+            s.stage("motion_field")
+        got = g.copy_all_solutions()
+        for n in ("dx", "dy", "dz"):
+            assert_same(got[n], o.get_field(n), f"level {i} {n}")
+        for n, kind in (("depth", "depth"), ("depth_old", "depth_old"), ("colour", "colour"),
+                        ("xx_old", "xx_old"), ("yy", "yy")):
+            np.testing.assert_array_equal(got[n], o.get_pyramid(kind, limg))
