@@ -60,7 +60,7 @@ def main():
     ap.add_argument("--tb-k", type=int, default=None,
                     help="iterations per launch of the temporally blocked kernel (1 = off)")
     ap.add_argument("--tb-min", type=int, default=None, help="smallest level (pixels) it runs on")
-    ap.add_argument("--workload", choices=("plain", "batched", "mixed"), default="plain",
+    ap.add_argument("--workload", choices=("plain", "batched", "mixed", "stream"), default="plain",
                     help="plain: one pair per step; batched: --batch-pairs pairs of --rows per "
                          "step; mixed: 128 pairs (50%% 480x640, 30%% 240x320, 20%% 120x160)")
     ap.add_argument("--batch-pairs", type=int, default=256)
@@ -81,6 +81,8 @@ def main():
     if args.rows not in (15, 30, 60, 120, 240, 480):
         print("bench.py: --rows must be one of 15, 30, 60, 120, 240, 480", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "stream":
+        return run_stream(args)
     if args.workload != "plain":
         return run_batched(args)
     rows, cam_mode = args.rows, 480 // args.rows   # input frames at the solve resolution
@@ -172,6 +174,46 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(rows, cam_mode, args.seed, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def run_stream(args):
+    """Streaming: one new frame per step (one pyramid per frame, the previous one reused),
+    then the solve of (previous frame, new frame)."""
+    from pdflow import Solver, level_table, synth_sequence_float
+    from pdflow.lib import DeviceBuffer
+    rows, cam = args.rows, 480 // args.rows
+    H, W = 480 // cam, 640 // cam
+    frames = []
+    for t in range(8):
+        I, Z = synth_sequence_float(args.seed, H, W, t)
+        bI, bZ = DeviceBuffer(I.nbytes), DeviceBuffer(Z.nbytes)
+        bI.upload(I)
+        bZ.upload(Z)
+        frames.append((bI, bZ))
+    s = Solver(rows, cam)
+    graph = not args.no_graph
+    n = 0
+    for _ in range(args.warmup + 1):
+        s.stream_frame_device(*frames[n % 8], graph=graph)
+        n += 1
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.stream_frame_device(*frames[n % 8], graph=graph)
+        n += 1
+    s.synchronize()
+    el = time.perf_counter() - t0
+    rate = args.steps / el
+    log(f"bench: streaming {H}x{W}: {args.steps} frames in {el:.3f} s -> {rate:.1f} flows/s")
+    table = level_table(rows, cam)
+    out = {"metric": METRIC, "value": round(rate, 3), "unit": "pairs/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": f"synthetic (seeded moving scene {args.seed}, 8 frames cycled)",
+           "config": {"workload": f"streaming plain PD-Flow {H}x{W}, rows={rows}: one new frame "
+                                  "and one pyramid per step, then the solve" + (", hipGraph" if graph else ""),
+                      "pixel_iterations_per_pair": int(sum(r * c * k for (_, r, c, _, k) in table))}}
     print(json.dumps(out), flush=True)
 
 
