@@ -45,6 +45,10 @@ int pdf_net_adam_step(pdf_net *n, float lr);
 /* forward + loss/backward + Adam step */
 int pdf_net_train_step(pdf_net *n, const float *x_dev, const float *target_dev, const float *mask_dev,
                        unsigned batch, float lr, float *loss);
+/* the same on samples idx_host[0..batch) of a device-resident data set of `total` samples laid
+ * out as x_all [total][h][w][4], t_all [total][h][w][3], m_all [total][h][w] */
+int pdf_net_train_step_indexed(pdf_net *n, const float *x_all, const float *t_all, const float *m_all,
+                               size_t total, const unsigned *idx_host, unsigned batch, float lr, float *loss);
 int pdf_net_synchronize(pdf_net *n);
 /* run on the caller's HIP stream (hipStream_t) from now on; the network's own stream is
  * released and the caller's is never destroyed by the library */

@@ -183,7 +183,7 @@ typedef struct pdf_batch pdf_batch;
 /* A batch of npairs independent pairs.  Pair i is solved at rows[i] x rows[i]*4/3 from input
  * frames of (640/cam_mode[i]) x (480/cam_mode[i]) (cam_mode NULL: 480/rows[i], the input at
  * the solve size), with the reference schedule for its level count (num_max_iter NULL) or
- * num_max_iter[i*6 + level], coarse first.  lambda_i, lambda_d, mu, the mask and fovh are
+ * num_max_iter[i*6 + level], coarse first (at most 100000 per level).  lambda_i, lambda_d, mu, the mask and fovh are
  * shared (readParameters, pdflow_cudalib.cu:176-193).  All memory is allocated here. */
 int pdf_batch_create(pdf_batch **out, unsigned npairs, const unsigned *rows, const unsigned *cam_mode,
                      float lambda_i, float lambda_d, float mu, const float *g_mask25, float fovh,
@@ -202,7 +202,8 @@ int pdf_batch_inputs_device(pdf_batch *b, unsigned pair, float **I1, float **Z1,
  * returned in *warm_device, which the caller fills before pdf_batch_solve.  Changing the
  * start level invalidates the recorded plan. */
 int pdf_batch_set_start_level(pdf_batch *b, unsigned pair, unsigned start_level, float **warm_device);
-/* per-pair iterations per level, coarse first (invalidates the plan when they change) */
+/* per-pair iterations per level, coarse first, at most 100000 each (invalidates the plan when
+ * they change) */
 int pdf_batch_set_schedule(pdf_batch *b, unsigned pair, const unsigned *num_max_iter);
 /* blocking upload of one pair's frames from host planes */
 int pdf_batch_set_inputs(pdf_batch *b, unsigned pair, const float *I1, const float *Z1,
@@ -221,6 +222,46 @@ int pdf_batch_launch_count(const pdf_batch *b, unsigned *launches);
  * eagerly while enabled) */
 int pdf_batch_enable_kernel_timing(pdf_batch *b, int enable);
 int pdf_batch_kernel_timing(pdf_batch *b, uint64_t *launches, double *total_ms, uint64_t *pixel_iters);
+/* planes of pair's last solve on the device: a field of include/pdflow_fields.h (those the
+ * batched path keeps, at the finest level) or a pyramid plane (kind, level) */
+int pdf_batch_field_device(pdf_batch *b, unsigned pair, int field, const float **ptr);
+int pdf_batch_pyramid_device(pdf_batch *b, unsigned pair, int kind, unsigned level, const float **ptr,
+                             unsigned *rows, unsigned *cols);
+
+/* ---- learned warm start and its fallback (include/pdflow_cnn.h; no reference
+ * counterpart) ------------------------------------------------------------------------ */
+
+struct pdf_net;
+/* The listed pairs (all if pairs is NULL) start at their level of the network's size (h x w
+ * of pdf_net_create), initialised from one batched forward pass of the network over those
+ * levels' I1, Z1, I2, Z2 inside the solve (same graph); each keeps its own schedule
+ * (pdf_batch_set_schedule).  net NULL switches every pair back to the plain start.  The
+ * network must outlive the batch's use of it and hold at least count images. */
+int pdf_batch_set_learned(pdf_batch *b, struct pdf_net *net, const unsigned *pairs, unsigned count);
+/* Per pair of the last solve: the finest level's energy on the last primal iterate,
+ *   E = sum |dct + dcu du + dcv dv| + mu_uv |ddt + ddu du + ddv dv - dw|
+ *       + lambda_i (|grad_r(du+du_prev)| + |grad_r(dv+dv_prev)|) + lambda_d |grad_r dw|
+ * (the terms of the prox .cu:906-923, the depth dual .cu:836-846, the TV duals .cu:849-885
+ * with grad_r of .cu:982-1013), and the warped-intensity residual: mean |I2(x + u) - I1(x)|
+ * over pixels with frame-1 depth, u the final flow (bilinear warp of .cu:654-682).
+ * Either output may be NULL. */
+int pdf_batch_metrics(pdf_batch *b, float *energy, float *residual);
+/* per pair: mean 3-D end-point error (metres) between this batch's flow and ref's over pixels
+ * with frame-1 depth (pairs of equal sizes) */
+int pdf_batch_epe(pdf_batch *b, const pdf_batch *ref, float *epe);
+/* re-solves the listed pairs in place with the plain path: level 0, the reference schedule for
+ * the pair's level count (whatever pdf_batch_set_schedule set), the pyramids already built */
+int pdf_batch_resolve_plain(pdf_batch *b, const unsigned *pairs, unsigned count);
+/* fallback rule: every pair whose energy or residual exceeds (1 + tol) times the reference
+ * value (e.g. the plain solver's, from pdf_batch_metrics of a plain batch) is re-solved by
+ * the plain path; flags[i] = 1 for those (flags may be NULL), *count their number */
+int pdf_batch_fallback(pdf_batch *b, const float *energy_ref, const float *residual_ref, float tol,
+                       unsigned char *flags, unsigned *count);
+/* training samples from a solved batch: for every pair, its h x w level's network input
+ * (x_dev + pair*h*w*4, NHWC as the network takes it), the label (t_dev + pair*h*w*3: the
+ * final flow averaged over each pixel's block, du, dv in pixels of that level, 10*dw) and
+ * the loss mask (m_dev + pair*h*w) */
+int pdf_batch_learn_samples(pdf_batch *b, unsigned h, unsigned w, float *x_dev, float *t_dev, float *m_dev);
 
 /* ---- device memory helpers for callers without a HIP binding ------------------------ */
 int pdf_device_alloc(void **ptr, size_t bytes);

@@ -36,6 +36,12 @@ int pdf_synth_pair(uint64_t seed, unsigned rows, unsigned cols,
 int pdf_synth_pair_float(uint64_t seed, unsigned rows, unsigned cols,
                          float *I1, float *Z1, float *I2, float *Z2, float *gt);
 
+/* n pairs with seeds seed0 .. seed0+n-1 rendered on `threads` host threads into consecutive
+ * rows*cols slots of each output (pair i at offset i*rows*cols; gt at i*3*rows*cols, may be
+ * NULL); same planes as pdf_synth_pair_float. */
+int pdf_synth_pairs_float(uint64_t seed0, unsigned n, unsigned rows, unsigned cols, float *I1,
+                          float *Z1, float *I2, float *Z2, float *gt, int threads);
+
 /* The zero-motion variant: frame 2 equals frame 1 exactly (no motion, no noise,
  * same holes).  Used by the zero-flow property tests. */
 int pdf_synth_static_float(uint64_t seed, unsigned rows, unsigned cols,

@@ -13,15 +13,24 @@
 // all pairs' two frames; per step four set-up launches and the iteration launches; tiles of all
 // pairs share one grid.  Ping-pong pointer swaps are resolved at plan time, so the sequence is
 // static and is captured once into a HIP graph.
+//
+// Learned warm start: pairs switched to it start at their level of the network's size; after
+// the pyramid the plan gathers those levels into one network input, runs the network's
+// forward pass (its kernels join the same graph) and injects the prediction as each pair's
+// du_prev, dv_prev, dw.  A fallback re-solves chosen pairs in place with the plain path (level
+// 0, the reference schedule) from the pyramids already built.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "../../include/pdflow_cnn.h"
 #include "../../include/pdflow_hip.h"
 #include "pdflow_internal.hpp"
+#include "pdflow_learn.hpp"
 
 namespace {
 
@@ -39,9 +48,15 @@ constexpr int kState[pdf::S_NUM] = {PDF_DU_NEW, PDF_DV_NEW, PDF_DW_NEW, PDF_DU_A
 
 std::string g_batch_create_error;
 
+// per-level iteration bound (the plan records one launch per few iterations)
+constexpr unsigned kMaxIters = 100000;
+
 struct Pair {
     unsigned rows, cols, cam_mode, width, height, ctf_levels, pyr_levels, lw;
     unsigned iters[PDF_MAX_CTF_LEVELS];
+    unsigned plain_iters[PDF_MAX_CTF_LEVELS];              // the reference schedule
+    bool learned;                                          // warm-started by the network
+    unsigned net_slot;                                     // its index in the network batch
     float *input[4];                                       // I1, Z1, I2, Z2
     unsigned start_level;                                  // 0, or a warm-started level
     float *warm;                                           // du, dv, dw planes of start_level
@@ -53,8 +68,9 @@ struct Pair {
 };
 
 struct Launch {
-    enum Kind { Setup, PD } kind;
+    enum Kind { Setup, PD, Learn, Net } kind;
     pdf::BatchKernel setup;
+    pdf::LearnKernel learn;
     int k;               // PD: 0 whole level, 1 single iteration, >1 temporal blocking
     int flag;
     size_t args_off, first_off;
@@ -63,6 +79,12 @@ struct Launch {
 };
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// a recorded launch sequence and its argument tables (host copy; device copy elsewhere)
+struct Plan {
+    std::vector<Launch> launches;
+    std::vector<char> tables;
+};
 
 }  // namespace
 
@@ -79,8 +101,8 @@ struct pdf_batch {
     bool level_kernel = true;
     // plan
     bool planned = false;
-    std::vector<Launch> launches;
-    std::vector<char> host_tables;
+    Plan plan;
+    std::vector<std::array<float *, 2 * pdf::S_NUM>> main_end;   // state pointers after the plan
     char *dev_tables = nullptr;
     size_t dev_tables_bytes = 0;
     hipGraphExec_t graph = nullptr;
@@ -90,6 +112,14 @@ struct pdf_batch {
     std::vector<uint64_t> event_pix;
     uint64_t timed_launches = 0, timed_pix = 0;
     double timed_ms = 0.0;
+    // learned warm start
+    pdf_net *net = nullptr;
+    unsigned net_h = 0, net_w = 0, net_count = 0;
+    float *learn_x = nullptr;
+    // scratch for the fallback plan, metric tables and partial sums (grown on demand)
+    char *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    Plan fb;
     std::string err;
 };
 
@@ -184,13 +214,13 @@ pdf::PyrArgs pyr_args(const pdf_batch *b, const Pair &p, unsigned i, bool old_fr
 }
 
 template <class T>
-void add_launch(pdf_batch *b, Launch l, const std::vector<T> &args, const std::vector<unsigned> &blocks)
+void add_launch(Plan &plan, Launch l, const std::vector<T> &args, const std::vector<unsigned> &blocks)
 {
     if (args.empty()) return;
     auto put = [&](const void *src, size_t bytes) {
-        const size_t off = align256(b->host_tables.size());
-        b->host_tables.resize(off + bytes);
-        std::memcpy(b->host_tables.data() + off, src, bytes);
+        const size_t off = align256(plan.tables.size());
+        plan.tables.resize(off + bytes);
+        std::memcpy(plan.tables.data() + off, src, bytes);
         return off;
     };
     std::vector<unsigned> first(args.size() + 1, 0);
@@ -199,50 +229,105 @@ void add_launch(pdf_batch *b, Launch l, const std::vector<T> &args, const std::v
     l.first_off = put(first.data(), first.size() * sizeof(unsigned));
     l.n = (unsigned)args.size();
     l.blocks = first.back();
-    b->launches.push_back(l);
+    plan.launches.push_back(l);
 }
 
-void build_plan(pdf_batch *b)
+Launch setup_launch(pdf::BatchKernel k, int flag)
 {
-    b->launches.clear();
-    b->host_tables.clear();
+    Launch l{};
+    l.kind = Launch::Setup;
+    l.setup = k;
+    l.flag = flag;
+    return l;
+}
+
+// the network-size level's inputs/outputs of a learned pair (plan-time view)
+pdf::LearnArgs learn_args(const pdf_batch *b, const Pair &p)
+{
+    unsigned r, c, limg;
+    level_dims(p, p.start_level, &r, &c, &limg);
+    pdf::LearnArgs a{};
+    a.I1 = p.pyr[PDF_PYR_COLOUR_OLD][limg];
+    a.Z1 = p.pyr[PDF_PYR_DEPTH_OLD][limg];
+    a.I2 = p.pyr[PDF_PYR_COLOUR][limg];
+    a.Z2 = p.pyr[PDF_PYR_DEPTH][limg];
+    a.rows = r;
+    a.cols = c;
+    a.x = b->learn_x + (size_t)p.net_slot * r * c * 4;
+    a.y = pdf::cnn::net_output(b->net) + (size_t)p.net_slot * r * c * 3;
+    a.warm = p.warm;
+    return a;
+}
+
+// Simulates a solve and records its launches.  who: the pairs solved; fallback: the plain
+// path from level 0 with the reference schedule and without the pyramid (already built).
+void build_plan(pdf_batch *b, const std::vector<unsigned> &who, bool fallback, Plan &plan)
+{
+    plan.launches.clear();
+    plan.tables.clear();
+    auto start_of = [&](const Pair &p) { return fallback ? 0u : p.start_level; };
+    auto iters_of = [&](const Pair &p, unsigned s) { return fallback ? p.plain_iters[s] : p.iters[s]; };
     unsigned max_pyr = 0, max_steps = 0;
-    for (const Pair &p : b->pairs) {
-        max_pyr = std::max(max_pyr, p.pyr_levels);
-        max_steps = std::max(max_steps, p.ctf_levels);
+    for (unsigned i : who) {
+        max_pyr = std::max(max_pyr, b->pairs[i].pyr_levels);
+        max_steps = std::max(max_steps, b->pairs[i].ctf_levels);
     }
     // both frames' pyramids, one launch per level (frame 1 -> "old", frame 2 -> "new")
-    for (unsigned l = 0; l < max_pyr; l++) {
+    for (unsigned l = 0; !fallback && l < max_pyr; l++) {
         std::vector<pdf::PyrArgs> args;
         std::vector<unsigned> blocks;
-        for (const Pair &p : b->pairs) {
+        for (unsigned i : who) {
+            const Pair &p = b->pairs[i];
             if (l >= p.pyr_levels) continue;
             for (int fr = 0; fr < 2; fr++) {
                 args.push_back(pyr_args(b, p, l, fr == 0));
                 blocks.push_back(pdf::batch_blocks_per_level(args.back().rows_i * args.back().cols_i));
             }
         }
-        Launch L{Launch::Setup, pdf::BatchKernel::Pyramid, 0, 0, 0, 0, 0, 0, 0};
-        add_launch(b, L, args, blocks);
+        add_launch(plan, setup_launch(pdf::BatchKernel::Pyramid, 0), args, blocks);
+    }
+    // learned pairs: network input from their start level, one forward pass, injection
+    if (!fallback && b->net) {
+        std::vector<pdf::LearnArgs> args;
+        std::vector<unsigned> blocks;
+        for (unsigned i : who) {
+            const Pair &p = b->pairs[i];
+            if (!p.learned) continue;
+            args.push_back(learn_args(b, p));
+            blocks.push_back(pdf::learn_blocks(args.back().rows * args.back().cols));
+        }
+        if (!args.empty()) {
+            Launch l{};
+            l.kind = Launch::Learn;
+            l.learn = pdf::LearnKernel::GatherInputs;
+            add_launch(plan, l, args, blocks);
+            Launch n{};
+            n.kind = Launch::Net;
+            n.n = b->net_count;
+            plan.launches.push_back(n);
+            l.learn = pdf::LearnKernel::Inject;
+            add_launch(plan, l, args, blocks);
+        }
     }
     for (unsigned s = 0; s < max_steps; s++) {
         std::vector<unsigned> active, warm_now, cold_now;
-        for (unsigned i = 0; i < b->pairs.size(); i++) {
+        for (unsigned i : who) {
             Pair &p = b->pairs[i];
-            if (s >= p.ctf_levels || s < p.start_level) continue;
+            const unsigned st = start_of(p);
+            if (s >= p.ctf_levels || s < st) continue;
             level_dims(p, s, &p.rows_i, &p.cols_i, &p.level_image);
             active.push_back(i);
-            (p.start_level > 0 && s == p.start_level ? warm_now : cold_now).push_back(i);
+            (st > 0 && s == st ? warm_now : cold_now).push_back(i);
         }
-        auto setup_for = [&](pdf::BatchKernel k, int flag, const std::vector<unsigned> &who) {
+        auto setup_for = [&](pdf::BatchKernel k, int flag, const std::vector<unsigned> &ids) {
             std::vector<pdf::LevelArgs> args;
             std::vector<unsigned> blocks;
-            for (unsigned i : who) {
+            for (unsigned i : ids) {
                 const Pair &p = b->pairs[i];
                 args.push_back(level_args(b, p));
                 blocks.push_back(pdf::batch_blocks_per_level(p.rows_i * p.cols_i));
             }
-            add_launch(b, Launch{Launch::Setup, k, 0, flag, 0, 0, 0, 0, 0}, args, blocks);
+            add_launch(plan, setup_launch(k, flag), args, blocks);
         };
         auto setup = [&](pdf::BatchKernel k, int flag) { setup_for(k, flag, active); };
         setup_for(pdf::BatchKernel::LevelInit, s > 0 ? 1 : 0, cold_now);
@@ -252,24 +337,28 @@ void build_plan(pdf_batch *b)
 
         // iterations: remaining count per active pair
         std::vector<unsigned> rem;
-        for (unsigned i : active) rem.push_back(b->pairs[i].iters[s]);
+        for (unsigned i : active) rem.push_back(iters_of(b->pairs[i], s));
         auto pd_round = [&](int k, auto take) {
             std::vector<pdf::PDArgs> args;
             std::vector<unsigned> blocks;
-            std::vector<unsigned> who;
+            std::vector<unsigned> ids;
             for (size_t j = 0; j < active.size(); j++) {
                 const unsigned n = take(rem[j]);
                 if (n == 0) continue;
                 Pair &p = b->pairs[active[j]];
                 args.push_back(pd_args(b, p, n));
                 blocks.push_back(pdf::pd_blocks(k, p.rows_i, p.cols_i));
-                who.push_back((unsigned)j);
+                ids.push_back((unsigned)j);
             }
             if (args.empty()) return false;
             uint64_t pix = 0;
             for (size_t q = 0; q < args.size(); q++) pix += (uint64_t)args[q].rows * args[q].cols * args[q].iters;
-            add_launch(b, Launch{Launch::PD, pdf::BatchKernel::LevelInit, k, 0, 0, 0, 0, 0, pix}, args, blocks);
-            for (unsigned j : who) {
+            Launch l{};
+            l.kind = Launch::PD;
+            l.k = k;
+            l.pixel_iters = pix;
+            add_launch(plan, l, args, blocks);
+            for (unsigned j : ids) {
                 rem[j] -= (unsigned)(k == 0 ? rem[j] : (k == 1 ? 1 : k));
                 swap_state(b->pairs[active[j]]);
             }
@@ -291,24 +380,70 @@ void build_plan(pdf_batch *b)
     }
 }
 
+void canonicalize(Pair &p)
+{
+    // canonical ping-pong assignment: f[] state planes are the lower addresses of each pair
+    for (int k = 0; k < pdf::S_NUM; k++)
+        if (p.f[kState[k]] > p.alt[kState[k]]) std::swap(p.f[kState[k]], p.alt[kState[k]]);
+}
+
+void save_state(const Pair &p, std::array<float *, 2 * pdf::S_NUM> &st)
+{
+    for (int k = 0; k < pdf::S_NUM; k++) {
+        st[k] = p.f[kState[k]];
+        st[pdf::S_NUM + k] = p.alt[kState[k]];
+    }
+}
+
+void load_state(Pair &p, const std::array<float *, 2 * pdf::S_NUM> &st)
+{
+    for (int k = 0; k < pdf::S_NUM; k++) {
+        p.f[kState[k]] = st[k];
+        p.alt[kState[k]] = st[pdf::S_NUM + k];
+    }
+}
+
+// device scratch of at least `bytes` (only outside graph capture)
+int ensure_scratch(pdf_batch *b, size_t bytes)
+{
+    if (b->scratch_bytes >= bytes) return PDF_OK;
+    BHIP(b, hipStreamSynchronize(b->stream));
+    if (b->scratch) { (void)hipFree(b->scratch); b->scratch = nullptr; b->scratch_bytes = 0; }
+    BHIP(b, hipMalloc((void **)&b->scratch, bytes));
+    b->scratch_bytes = bytes;
+    return PDF_OK;
+}
+
 int ensure_plan(pdf_batch *b)
 {
     if (b->planned) return PDF_OK;
     // the plan mutates the pairs' ping-pong pointers; it starts from the canonical state
     // and ends in the state the solve leaves, which is what the accessors report
-    build_plan(b);
+    std::vector<unsigned> all(b->pairs.size());
+    for (unsigned i = 0; i < all.size(); i++) {
+        all[i] = i;
+        canonicalize(b->pairs[i]);
+    }
+    build_plan(b, all, false, b->plan);
+    b->main_end.resize(b->pairs.size());
+    for (size_t i = 0; i < b->pairs.size(); i++) save_state(b->pairs[i], b->main_end[i]);
     if (b->dev_tables) { (void)hipFree(b->dev_tables); b->dev_tables = nullptr; }
-    b->dev_tables_bytes = b->host_tables.size();
+    b->dev_tables_bytes = b->plan.tables.size();
     BHIP(b, hipMalloc((void **)&b->dev_tables, b->dev_tables_bytes));
-    BHIP(b, hipMemcpy(b->dev_tables, b->host_tables.data(), b->dev_tables_bytes, hipMemcpyHostToDevice));
+    BHIP(b, hipMemcpy(b->dev_tables, b->plan.tables.data(), b->dev_tables_bytes, hipMemcpyHostToDevice));
     b->planned = true;
     return PDF_OK;
 }
 
-int run_plan(pdf_batch *b, bool timed)
+int run_plan(pdf_batch *b, const Plan &plan, char *dev_tables, bool timed)
 {
-    for (const Launch &l : b->launches) {
-        pdf::BatchTable t{b->dev_tables + l.args_off, (const unsigned *)(b->dev_tables + l.first_off), l.n};
+    for (const Launch &l : plan.launches) {
+        pdf::BatchTable t{dev_tables + l.args_off, (const unsigned *)(dev_tables + l.first_off), l.n};
+        if (l.kind == Launch::Net) {
+            const int rc = pdf::cnn::net_forward_on(b->net, b->learn_x, l.n, b->stream);
+            if (rc) return bfail(b, rc, std::string("network forward: ") + pdf_net_last_error(b->net));
+            continue;
+        }
         const bool ev = timed && l.kind == Launch::PD;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (ev) {
@@ -316,9 +451,12 @@ int run_plan(pdf_batch *b, bool timed)
             BHIP(b, hipEventCreate(&e1));
             BHIP(b, hipEventRecord(e0, b->stream));
         }
-        const hipError_t e = l.kind == Launch::Setup
-                                 ? pdf::launch_batch(l.setup, t, l.blocks, l.flag, b->stream)
-                                 : pdf::launch_pd_batch(l.k, t, l.blocks, b->stream);
+        hipError_t e;
+        switch (l.kind) {
+        case Launch::Setup: e = pdf::launch_batch(l.setup, t, l.blocks, l.flag, b->stream); break;
+        case Launch::PD: e = pdf::launch_pd_batch(l.k, t, l.blocks, b->stream); break;
+        default: e = pdf::launch_learn(l.learn, t, l.blocks, nullptr, b->stream); break;
+        }
         if (e != hipSuccess) return bfail(b, PDF_ERR_HIP, std::string("batched launch: ") + hipGetErrorString(e));
         if (ev) {
             BHIP(b, hipEventRecord(e1, b->stream));
@@ -350,16 +488,32 @@ int finish_timing(pdf_batch *b)
 void drop_plan(pdf_batch *b)
 {
     if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
-    if (b->planned) {
-        // restore canonical ping-pong assignment: an odd number of swaps per pair leaves
-        // f/alt exchanged relative to the arena layout
-        for (Pair &p : b->pairs) {
-            // canonical: f[] state planes are the lower addresses of each state pair
-            for (int k = 0; k < pdf::S_NUM; k++)
-                if (p.f[kState[k]] > p.alt[kState[k]]) std::swap(p.f[kState[k]], p.alt[kState[k]]);
-        }
-    }
+    for (Pair &p : b->pairs) canonicalize(p);
     b->planned = false;
+}
+
+// a one-off plan (fallback, reductions) in the scratch buffer: [tables | extra bytes]
+int upload_scratch_plan(pdf_batch *b, const Plan &plan, size_t extra, char **tables, char **extra_ptr)
+{
+    const size_t tb = align256(plan.tables.size());
+    int rc = ensure_scratch(b, tb + extra + 256);
+    if (rc) return rc;
+    BHIP(b, hipStreamSynchronize(b->stream));
+    BHIP(b, hipMemcpy(b->scratch, plan.tables.data(), plan.tables.size(), hipMemcpyHostToDevice));
+    *tables = b->scratch;
+    if (extra_ptr) *extra_ptr = b->scratch + tb;
+    return PDF_OK;
+}
+
+// the level of pair p with the given size, or -1
+int level_of_size(const Pair &p, unsigned h, unsigned w)
+{
+    for (unsigned s = 0; s < p.ctf_levels; s++) {
+        unsigned r, c, limg;
+        level_dims(p, s, &r, &c, &limg);
+        if (r == h && c == w) return (int)s;
+    }
+    return -1;
 }
 
 }  // namespace
@@ -404,11 +558,21 @@ int pdf_batch_create(pdf_batch **out, unsigned npairs, const unsigned *rows, con
         p.lw = 0;
         while ((p.cols << (p.lw + 1)) <= p.width) p.lw++;
         p.pyr_levels = p.lw + p.ctf_levels;
+        // the reference schedule (scene_flow_impair.cpp:46-52): 100 at the finest level, 15
+        // fewer per coarser one; it is also what the plain fallback re-solves with
+        unsigned t[6];
+        for (int l = 5; l >= 0; l--) t[l] = (l >= (int)p.ctf_levels - 1) ? 100u : t[l + 1] - 15u;
+        std::memcpy(p.plain_iters, t, sizeof(t));
         if (num_max_iter) {
-            for (unsigned l = 0; l < p.ctf_levels; l++) p.iters[l] = num_max_iter[i * PDF_MAX_CTF_LEVELS + l];
+            for (unsigned l = 0; l < p.ctf_levels; l++) {
+                p.iters[l] = num_max_iter[i * PDF_MAX_CTF_LEVELS + l];
+                if (p.iters[l] > kMaxIters) {
+                    g_batch_create_error = "pair " + std::to_string(i) + ": more than 100000 iterations at a level";
+                    delete b;
+                    return PDF_ERR_ARG;
+                }
+            }
         } else {
-            unsigned t[6];
-            for (int l = 5; l >= 0; l--) t[l] = (l >= (int)p.ctf_levels - 1) ? 100u : t[l + 1] - 15u;
             std::memcpy(p.iters, t, sizeof(t));
         }
         const size_t P0 = (size_t)p.width * p.height;
@@ -457,6 +621,8 @@ int pdf_batch_destroy(pdf_batch *b)
     for (auto &ev : b->events) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     if (b->dev_tables) (void)hipFree(b->dev_tables);
+    if (b->scratch) (void)hipFree(b->scratch);
+    if (b->learn_x) (void)hipFree(b->learn_x);
     if (b->own_stream) (void)hipStreamDestroy(b->stream);
     (void)hipFree(b->arena);
     delete b;
@@ -486,9 +652,16 @@ int pdf_batch_set_start_level(pdf_batch *b, unsigned pair, unsigned start_level,
     if (!b || pair >= b->pairs.size()) return bfail(b, PDF_ERR_ARG, "bad pair index");
     Pair &p = b->pairs[pair];
     if (start_level >= p.ctf_levels) return bfail(b, PDF_ERR_ARG, "start level beyond the finest level");
-    if (p.start_level != start_level) {
+    if (p.start_level != start_level || p.learned) {
         drop_plan(b);
         p.start_level = start_level;
+        if (p.learned) {
+            // the pair leaves the network batch: renumber the others' slots
+            p.learned = false;
+            b->net_count = 0;
+            for (Pair &q : b->pairs)
+                if (q.learned) q.net_slot = b->net_count++;
+        }
     }
     if (warm_device) *warm_device = p.warm;
     return PDF_OK;
@@ -498,6 +671,8 @@ int pdf_batch_set_schedule(pdf_batch *b, unsigned pair, const unsigned *num_max_
 {
     if (!b || pair >= b->pairs.size() || !num_max_iter) return bfail(b, PDF_ERR_ARG, "bad pair index or schedule");
     Pair &p = b->pairs[pair];
+    for (unsigned l = 0; l < p.ctf_levels; l++)
+        if (num_max_iter[l] > kMaxIters) return bfail(b, PDF_ERR_ARG, "more than 100000 iterations at a level");
     if (std::memcmp(p.iters, num_max_iter, p.ctf_levels * sizeof(unsigned)) != 0) {
         drop_plan(b);
         std::memcpy(p.iters, num_max_iter, p.ctf_levels * sizeof(unsigned));
@@ -534,10 +709,12 @@ int pdf_batch_solve(pdf_batch *b, int use_graph)
     if (!b) return PDF_ERR_ARG;
     int rc = ensure_plan(b);
     if (rc) return rc;
-    if (!use_graph || b->timing) return run_plan(b, b->timing);
+    // every pair ends in the state the plan leaves (a fallback may have changed some)
+    for (size_t i = 0; i < b->pairs.size(); i++) load_state(b->pairs[i], b->main_end[i]);
+    if (!use_graph || b->timing) return run_plan(b, b->plan, b->dev_tables, b->timing);
     if (!b->graph) {
         BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        rc = run_plan(b, false);
+        rc = run_plan(b, b->plan, b->dev_tables, false);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(b->stream, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -609,7 +786,244 @@ int pdf_batch_copy_flow(pdf_batch *b, unsigned pair, float *dx, float *dy, float
 int pdf_batch_launch_count(const pdf_batch *b, unsigned *launches)
 {
     if (!b || !launches) return PDF_ERR_ARG;
-    *launches = (unsigned)b->launches.size();
+    *launches = (unsigned)b->plan.launches.size();
+    return PDF_OK;
+}
+
+int pdf_batch_set_learned(pdf_batch *b, pdf_net *net, const unsigned *pairs, unsigned count)
+{
+    if (!b) return PDF_ERR_ARG;
+    drop_plan(b);
+    // switch every learned pair back to a cold start first
+    for (Pair &p : b->pairs)
+        if (p.learned) {
+            p.learned = false;
+            p.start_level = 0;
+        }
+    b->net = nullptr;
+    b->net_count = 0;
+    if (!net) return PDF_OK;
+    unsigned h, w, maxb;
+    pdf::cnn::net_dims(net, &h, &w, &maxb);
+    std::vector<unsigned> who;
+    if (pairs) who.assign(pairs, pairs + count);
+    else for (unsigned i = 0; i < b->pairs.size(); i++) who.push_back(i);
+    if (who.size() > maxb) return bfail(b, PDF_ERR_ARG, "more learned pairs than the network's max_batch");
+    std::vector<int> level(who.size());
+    for (size_t j = 0; j < who.size(); j++) {
+        if (who[j] >= b->pairs.size()) return bfail(b, PDF_ERR_ARG, "bad pair index");
+        level[j] = level_of_size(b->pairs[who[j]], h, w);
+        if (level[j] < 0)
+            return bfail(b, PDF_ERR_ARG, "pair " + std::to_string(who[j]) + " has no level of the network's size");
+    }
+    if (b->learn_x) { (void)hipFree(b->learn_x); b->learn_x = nullptr; }
+    if (!who.empty()) BHIP(b, hipMalloc((void **)&b->learn_x, who.size() * (size_t)h * w * 4 * sizeof(float)));
+    for (size_t j = 0; j < who.size(); j++) {
+        Pair &p = b->pairs[who[j]];
+        if (p.learned) return bfail(b, PDF_ERR_ARG, "pair listed twice");
+        p.learned = true;
+        p.start_level = (unsigned)level[j];
+        p.net_slot = (unsigned)j;
+    }
+    b->net = net;
+    b->net_h = h;
+    b->net_w = w;
+    b->net_count = (unsigned)who.size();
+    return PDF_OK;
+}
+
+int pdf_batch_resolve_plain(pdf_batch *b, const unsigned *pairs, unsigned count)
+{
+    if (!b || (count && !pairs)) return PDF_ERR_ARG;
+    if (!b->planned) return bfail(b, PDF_ERR_STATE, "re-solve before any solve");
+    if (count == 0) return PDF_OK;
+    std::vector<unsigned> who(pairs, pairs + count);
+    for (unsigned i : who)
+        if (i >= b->pairs.size()) return bfail(b, PDF_ERR_ARG, "bad pair index");
+    for (unsigned i : who) canonicalize(b->pairs[i]);
+    build_plan(b, who, true, b->fb);
+    char *tables = nullptr;
+    int rc = upload_scratch_plan(b, b->fb, 0, &tables, nullptr);
+    if (rc) return rc;
+    return run_plan(b, b->fb, tables, false);
+}
+
+int pdf_batch_metrics(pdf_batch *b, float *energy, float *residual)
+{
+    if (!b) return PDF_ERR_ARG;
+    if (!b->planned) return bfail(b, PDF_ERR_STATE, "metrics before any solve");
+    Plan m;
+    std::vector<pdf::MetricArgs> args;
+    std::vector<unsigned> blocks;
+    for (const Pair &p : b->pairs) {
+        const unsigned L = p.lw;   // the finest level's pyramid image
+        pdf::MetricArgs a{};
+        a.dct = p.f[PDF_DCT]; a.dcu = p.f[PDF_DCU]; a.dcv = p.f[PDF_DCV];
+        a.ddt = p.f[PDF_DDT]; a.ddu = p.f[PDF_DDU]; a.ddv = p.f[PDF_DDV];
+        a.mu_uv = p.f[PDF_MU_UV]; a.ri = p.f[PDF_RI]; a.rj = p.f[PDF_RJ];
+        a.du_prev = p.f[PDF_DU_PREV]; a.dv_prev = p.f[PDF_DV_PREV];
+        a.du = p.f[PDF_DU_NEW]; a.dv = p.f[PDF_DV_NEW]; a.dw = p.f[PDF_DW_NEW];
+        a.I1 = p.pyr[PDF_PYR_COLOUR_OLD][L]; a.I2 = p.pyr[PDF_PYR_COLOUR][L]; a.Zold = p.pyr[PDF_PYR_DEPTH_OLD][L];
+        a.du_l = p.f[PDF_DU_L]; a.dv_l = p.f[PDF_DV_L];
+        a.rows = p.rows; a.cols = p.cols;
+        a.lambda_i = b->lambda_i; a.lambda_d = b->lambda_d;
+        args.push_back(a);
+        blocks.push_back(pdf::learn_blocks(p.rows * p.cols));
+    }
+    Launch l{};
+    l.kind = Launch::Learn;
+    l.learn = pdf::LearnKernel::Metrics;
+    add_launch(m, l, args, blocks);
+    const Launch &L = m.launches[0];
+    char *tables = nullptr, *extra = nullptr;
+    int rc = upload_scratch_plan(b, m, 3 * sizeof(float) * L.blocks, &tables, &extra);
+    if (rc) return rc;
+    pdf::BatchTable t{tables + L.args_off, (const unsigned *)(tables + L.first_off), L.n};
+    hipError_t e = pdf::launch_learn(L.learn, t, L.blocks, (float *)extra, b->stream);
+    if (e != hipSuccess) return bfail(b, PDF_ERR_HIP, std::string("metrics launch: ") + hipGetErrorString(e));
+    std::vector<float> part(3 * (size_t)L.blocks);
+    BHIP(b, hipMemcpyAsync(part.data(), extra, part.size() * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    size_t blk = 0;
+    for (size_t i = 0; i < b->pairs.size(); i++) {
+        double es = 0, rs = 0, rc_ = 0;
+        for (unsigned q = 0; q < blocks[i]; q++, blk++) {
+            es += part[3 * blk];
+            rs += part[3 * blk + 1];
+            rc_ += part[3 * blk + 2];
+        }
+        if (energy) energy[i] = (float)es;
+        if (residual) residual[i] = (float)(rc_ > 0 ? rs / rc_ : 0.0);
+    }
+    return PDF_OK;
+}
+
+int pdf_batch_epe(pdf_batch *b, const pdf_batch *ref, float *epe)
+{
+    if (!b || !ref || !epe) return PDF_ERR_ARG;
+    if (ref->pairs.size() != b->pairs.size()) return bfail(b, PDF_ERR_ARG, "batches of different sizes");
+    Plan m;
+    std::vector<pdf::EpeArgs> args;
+    std::vector<unsigned> blocks;
+    for (size_t i = 0; i < b->pairs.size(); i++) {
+        const Pair &p = b->pairs[i], &q = ref->pairs[i];
+        if (p.rows != q.rows || p.cols != q.cols) return bfail(b, PDF_ERR_ARG, "pairs of different sizes");
+        pdf::EpeArgs a{p.f[PDF_DX], p.f[PDF_DY], p.f[PDF_DZ], q.f[PDF_DX], q.f[PDF_DY], q.f[PDF_DZ],
+                       p.pyr[PDF_PYR_DEPTH_OLD][p.lw], p.rows * p.cols};
+        args.push_back(a);
+        blocks.push_back(pdf::learn_blocks(p.rows * p.cols));
+    }
+    Launch l{};
+    l.kind = Launch::Learn;
+    l.learn = pdf::LearnKernel::Epe;
+    add_launch(m, l, args, blocks);
+    const Launch &L = m.launches[0];
+    char *tables = nullptr, *extra = nullptr;
+    // the reference batch's work must be complete too
+    BHIP(b, hipStreamSynchronize(ref->stream));
+    int rc = upload_scratch_plan(b, m, 2 * sizeof(float) * L.blocks, &tables, &extra);
+    if (rc) return rc;
+    pdf::BatchTable t{tables + L.args_off, (const unsigned *)(tables + L.first_off), L.n};
+    hipError_t e = pdf::launch_learn(L.learn, t, L.blocks, (float *)extra, b->stream);
+    if (e != hipSuccess) return bfail(b, PDF_ERR_HIP, std::string("EPE launch: ") + hipGetErrorString(e));
+    std::vector<float> part(2 * (size_t)L.blocks);
+    BHIP(b, hipMemcpyAsync(part.data(), extra, part.size() * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    size_t blk = 0;
+    for (size_t i = 0; i < b->pairs.size(); i++) {
+        double s = 0, c = 0;
+        for (unsigned q = 0; q < blocks[i]; q++, blk++) {
+            s += part[2 * blk];
+            c += part[2 * blk + 1];
+        }
+        epe[i] = (float)(c > 0 ? s / c : 0.0);
+    }
+    return PDF_OK;
+}
+
+int pdf_batch_fallback(pdf_batch *b, const float *energy_ref, const float *residual_ref, float tol,
+                       unsigned char *flags, unsigned *count)
+{
+    if (!b || !energy_ref || !residual_ref || tol < 0.f) return PDF_ERR_ARG;
+    const size_t n = b->pairs.size();
+    std::vector<float> e(n), r(n);
+    int rc = pdf_batch_metrics(b, e.data(), r.data());
+    if (rc) return rc;
+    std::vector<unsigned> who;
+    for (size_t i = 0; i < n; i++) {
+        const bool bad = !(e[i] <= (1.f + tol) * energy_ref[i]) || !(r[i] <= (1.f + tol) * residual_ref[i]);
+        if (flags) flags[i] = bad ? 1 : 0;
+        if (bad) who.push_back((unsigned)i);
+    }
+    if (count) *count = (unsigned)who.size();
+    return pdf_batch_resolve_plain(b, who.data(), (unsigned)who.size());
+}
+
+int pdf_batch_learn_samples(pdf_batch *b, unsigned h, unsigned w, float *x_dev, float *t_dev, float *m_dev)
+{
+    if (!b || !x_dev || !t_dev || !m_dev || h == 0 || w == 0) return PDF_ERR_ARG;
+    if (!b->planned) return bfail(b, PDF_ERR_STATE, "samples before any solve");
+    Plan m;
+    std::vector<pdf::LearnArgs> args;
+    std::vector<unsigned> blocks;
+    for (size_t i = 0; i < b->pairs.size(); i++) {
+        const Pair &p = b->pairs[i];
+        const int s = level_of_size(p, h, w);
+        if (s < 0) return bfail(b, PDF_ERR_ARG, "pair " + std::to_string(i) + " has no level of that size");
+        unsigned r, c, limg;
+        level_dims(p, (unsigned)s, &r, &c, &limg);
+        pdf::LearnArgs a{};
+        a.I1 = p.pyr[PDF_PYR_COLOUR_OLD][limg];
+        a.Z1 = p.pyr[PDF_PYR_DEPTH_OLD][limg];
+        a.I2 = p.pyr[PDF_PYR_COLOUR][limg];
+        a.Z2 = p.pyr[PDF_PYR_DEPTH][limg];
+        a.rows = h;
+        a.cols = w;
+        a.x = x_dev + i * (size_t)h * w * 4;
+        a.du = p.f[PDF_DU_L];
+        a.dv = p.f[PDF_DV_L];
+        a.dw = p.f[PDF_DW_L];
+        a.frows = p.rows;
+        a.fcols = p.cols;
+        a.t = t_dev + i * (size_t)h * w * 3;
+        a.m = m_dev + i * (size_t)h * w;
+        args.push_back(a);
+        blocks.push_back(pdf::learn_blocks(h * w));
+    }
+    Launch l{};
+    l.kind = Launch::Learn;
+    l.learn = pdf::LearnKernel::GatherInputs;
+    add_launch(m, l, args, blocks);
+    char *tables = nullptr;
+    int rc = upload_scratch_plan(b, m, 0, &tables, nullptr);
+    if (rc) return rc;
+    const Launch &L = m.launches[0];
+    pdf::BatchTable t{tables + L.args_off, (const unsigned *)(tables + L.first_off), L.n};
+    hipError_t e = pdf::launch_learn(pdf::LearnKernel::GatherInputs, t, L.blocks, nullptr, b->stream);
+    if (e == hipSuccess) e = pdf::launch_learn(pdf::LearnKernel::Labels, t, L.blocks, nullptr, b->stream);
+    if (e != hipSuccess) return bfail(b, PDF_ERR_HIP, std::string("sample launch: ") + hipGetErrorString(e));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return PDF_OK;
+}
+
+int pdf_batch_field_device(pdf_batch *b, unsigned pair, int field, const float **ptr)
+{
+    if (!b || pair >= b->pairs.size() || !ptr || field < 0 || field >= PDF_NUM_FIELDS)
+        return bfail(b, PDF_ERR_ARG, "bad pair index or field");
+    *ptr = b->pairs[pair].f[field];
+    return *ptr ? PDF_OK : bfail(b, PDF_ERR_ARG, "field not kept by the batched solver");
+}
+
+int pdf_batch_pyramid_device(pdf_batch *b, unsigned pair, int kind, unsigned level, const float **ptr,
+                             unsigned *rows, unsigned *cols)
+{
+    if (!b || pair >= b->pairs.size() || !ptr || kind < 0 || kind >= PDF_NUM_PYR_KINDS)
+        return bfail(b, PDF_ERR_ARG, "bad pair index or pyramid kind");
+    const Pair &p = b->pairs[pair];
+    if (level >= p.pyr_levels) return bfail(b, PDF_ERR_ARG, "pyramid level out of range");
+    *ptr = p.pyr[kind][level];
+    if (rows) *rows = p.height >> level;
+    if (cols) *cols = p.width >> level;
     return PDF_OK;
 }
 

@@ -353,6 +353,15 @@ __global__ void k_adam(float *w, const float *g, float *m, float *v, size_t n, f
     w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
 }
 
+// dst row r = src row idx[r] (rows of row4 float4s)
+__global__ void k_gather_rows(const float4 *src, const unsigned *idx, float4 *dst, unsigned count, size_t row4)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= count * row4) return;
+    const size_t r = i / row4, c = i % row4;
+    dst[i] = src[(size_t)idx[r] * row4 + c];
+}
+
 }  // namespace
 
 inline dim3 g1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
@@ -438,6 +447,15 @@ hipError_t l1_loss(const float *out, const float *target, const float *mask, uin
                    float scale, hipStream_t s)
 {
     hipLaunchKernelGGL(k_l1, g1(P), dim3(256), 0, s, out, target, mask, grad, partial, P, scale);
+    return hipGetLastError();
+}
+
+hipError_t gather_rows(const float *src, const unsigned *idx, float *dst, unsigned count, size_t row_floats,
+                       hipStream_t s)
+{
+    const size_t row4 = row_floats / 4;
+    hipLaunchKernelGGL(k_gather_rows, g1(count * row4), dim3(256), 0, s, (const float4 *)src, idx, (float4 *)dst, count,
+                       row4);
     return hipGetLastError();
 }
 

@@ -23,6 +23,7 @@
 #include "../../include/pdflow_cnn.h"
 #include "../../include/pdflow_hip.h"
 #include "pdflow_cnn_internal.hpp"
+#include "pdflow_learn.hpp"
 
 namespace {
 
@@ -67,6 +68,9 @@ struct pdf_net {
     uint16_t *gout = nullptr;    // output gradient [P][16]
     float *out = nullptr;        // [P][3]
     float *partial = nullptr;
+    // staging of indexed training batches
+    float *xs = nullptr, *ts = nullptr, *ms = nullptr;
+    unsigned *idx = nullptr;
     std::vector<char *> allocs;
     hipStream_t stream = nullptr;
     bool own_stream = true;      // false after pdf_net_set_stream with a caller's stream
@@ -218,6 +222,35 @@ int check_batch(pdf_net *n, unsigned nb)
 
 }  // namespace
 
+namespace pdf {
+namespace cnn {
+
+int net_forward_on(pdf_net *n, const float *x_dev, unsigned nb, hipStream_t s)
+{
+    int rc = check_batch(n, nb);
+    if (rc) return rc;
+    hipStream_t own = n->stream;
+    n->stream = s;
+    n->prepared = false;
+    const size_t P = (size_t)nb * n->h * n->w;
+    hipError_t e = f32_to_bf16(x_dev, n->x0, P * 4, s);
+    rc = e == hipSuccess ? run_forward(n, (int)nb) : nfail(n, PDF_ERR_HIP, hipGetErrorString(e));
+    n->stream = own;
+    return rc;
+}
+
+const float *net_output(const pdf_net *n) { return n->out; }
+
+void net_dims(const pdf_net *n, unsigned *h, unsigned *w, unsigned *max_batch)
+{
+    *h = (unsigned)n->h;
+    *w = (unsigned)n->w;
+    *max_batch = (unsigned)n->max_batch;
+}
+
+}  // namespace cnn
+}  // namespace pdf
+
 extern "C" {
 
 const char *pdf_net_last_error(const pdf_net *n) { return n ? n->err.c_str() : g_net_error.c_str(); }
@@ -290,6 +323,10 @@ int pdf_net_create(pdf_net **out, unsigned max_batch, unsigned h, unsigned w, ui
     NALLOC(n->gout, P1 * 16);
     NALLOC(n->out, P1 * 3);
     NALLOC(n->partial, 2 * ((P1 + 255) / 256));
+    NALLOC(n->xs, P1 * 4);
+    NALLOC(n->ts, P1 * 3);
+    NALLOC(n->ms, P1);
+    NALLOC(n->idx, max_batch);
 #undef NALLOC
     e = hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail_alloc(e);
@@ -447,6 +484,22 @@ int pdf_net_train_step(pdf_net *n, const float *x_dev, const float *target_dev, 
     if (!rc) rc = pdf_net_loss_backward(n, target_dev, mask_dev, nb, loss);
     if (!rc) rc = pdf_net_adam_step(n, lr);
     return rc;
+}
+
+int pdf_net_train_step_indexed(pdf_net *n, const float *x_all, const float *t_all, const float *m_all, size_t total,
+                               const unsigned *idx_host, unsigned nb, float lr, float *loss)
+{
+    int rc = check_batch(n, nb);
+    if (rc) return rc;
+    if (!x_all || !t_all || !m_all || !idx_host) return nfail(n, PDF_ERR_ARG, "null argument");
+    for (unsigned i = 0; i < nb; i++)
+        if (idx_host[i] >= total) return nfail(n, PDF_ERR_ARG, "sample index beyond the data set");
+    const size_t hw = (size_t)n->h * n->w;
+    NHIP(n, hipMemcpyAsync(n->idx, idx_host, nb * sizeof(unsigned), hipMemcpyHostToDevice, n->stream));
+    NHIP(n, gather_rows(x_all, n->idx, n->xs, nb, hw * 4, n->stream));
+    NHIP(n, gather_rows(t_all, n->idx, n->ts, nb, hw * 3, n->stream));
+    NHIP(n, gather_rows(m_all, n->idx, n->ms, nb, hw, n->stream));
+    return pdf_net_train_step(n, n->xs, n->ts, n->ms, nb, lr, loss);
 }
 
 int pdf_net_synchronize(pdf_net *n)

@@ -43,6 +43,9 @@ hipError_t prepare_weights(const float *wm, uint16_t *wt, uint16_t *wd, int cin,
                            hipStream_t s);
 hipError_t l1_loss(const float *out, const float *target, const float *mask, uint16_t *grad, float *partial, int P,
                    float scale, hipStream_t s);
+// dst row r = src row idx[r]; row_floats a multiple of 4, src/dst 16-byte aligned
+hipError_t gather_rows(const float *src, const unsigned *idx, float *dst, unsigned count, size_t row_floats,
+                       hipStream_t s);
 hipError_t adam(float *w, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
                 float bc1, float bc2, float gscale, hipStream_t s);
 

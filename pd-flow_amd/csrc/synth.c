@@ -341,3 +341,17 @@ uint64_t pdf_synth_hash(const void *data, uint64_t nbytes)
     }
     return h;
 }
+
+int pdf_synth_pairs_float(uint64_t seed0, unsigned n, unsigned rows, unsigned cols, float *I1,
+                          float *Z1, float *I2, float *Z2, float *gt, int threads)
+{
+    const size_t P = (size_t)rows * cols;
+    int rc = 0;
+    if (threads <= 0) threads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads) reduction(| : rc)
+    for (long i = 0; i < (long)n; i++)
+        rc |= pdf_synth_pair_float(seed0 + (uint64_t)i, rows, cols, I1 ? I1 + i * P : NULL,
+                                   Z1 ? Z1 + i * P : NULL, I2 ? I2 + i * P : NULL,
+                                   Z2 ? Z2 + i * P : NULL, gt ? gt + 3 * i * P : NULL);
+    return rc;
+}

@@ -119,6 +119,14 @@ def hip_library():
         "pdf_solve_warm": ([vp, up, c_uint, vp], c_int),
         "pdf_stream_frame_device": ([vp, vp, vp, up, c_int, ctypes.POINTER(c_int)], c_int),
         "pdf_batch_enable_kernel_timing": ([vp, c_int], c_int),
+        "pdf_batch_field_device": ([vp, c_uint, c_int, ctypes.POINTER(vp)], c_int),
+        "pdf_batch_pyramid_device": ([vp, c_uint, c_int, c_uint, ctypes.POINTER(vp), up_, up_], c_int),
+        "pdf_batch_set_learned": ([vp, vp, up_, c_uint], c_int),
+        "pdf_batch_metrics": ([vp, fp, fp], c_int),
+        "pdf_batch_epe": ([vp, vp, fp], c_int),
+        "pdf_batch_resolve_plain": ([vp, up_, c_uint], c_int),
+        "pdf_batch_fallback": ([vp, fp, fp, c_float, ctypes.POINTER(ctypes.c_ubyte), up_], c_int),
+        "pdf_batch_learn_samples": ([vp, c_uint, c_uint, vp, vp, vp], c_int),
         "pdf_batch_kernel_timing": ([vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_uint64)], c_int),
     })
@@ -350,6 +358,8 @@ class Batch:
         self.cam_modes = list(cam_modes) if cam_modes is not None else [480 // r for r in self.rows]
         it_arr = None
         if iters is not None:
+            if len(iters) != n:
+                raise ValueError(f"one schedule per pair expected ({n}), got {len(iters)}")
             flat = []
             for sched in iters:
                 flat += list(sched) + [0] * (6 - len(sched))
@@ -423,3 +433,72 @@ class Batch:
         n = ctypes.c_uint()
         self._chk(self.lib.pdf_batch_launch_count(self.ctx, ctypes.byref(n)))
         return n.value
+
+    def field_device(self, pair, name):
+        ptr = ctypes.c_void_p()
+        self._chk(self.lib.pdf_batch_field_device(self.ctx, pair, FIELDS[name], ctypes.byref(ptr)))
+        return ptr.value
+
+    def field(self, pair, name):
+        """a finest-level plane of pair's last solve, copied to the host"""
+        r = self.rows[pair]
+        out = np.empty(r * (r * 4 // 3), np.float32)
+        self.synchronize()
+        rc = self.lib.pdf_memcpy_d2h(out.ctypes.data, self.field_device(pair, name), out.nbytes)
+        if rc:
+            raise PDFlowError(rc, "device-to-host copy")
+        return out
+
+    def pyramid(self, pair, kind, level):
+        ptr, r, c = ctypes.c_void_p(), ctypes.c_uint(), ctypes.c_uint()
+        self._chk(self.lib.pdf_batch_pyramid_device(self.ctx, pair, PYR_KINDS[kind], level, ctypes.byref(ptr),
+                                                    ctypes.byref(r), ctypes.byref(c)))
+        out = np.empty(r.value * c.value, np.float32)
+        self.synchronize()
+        rc = self.lib.pdf_memcpy_d2h(out.ctypes.data, ptr, out.nbytes)
+        if rc:
+            raise PDFlowError(rc, "device-to-host copy")
+        return out, r.value, c.value
+
+    def set_learned(self, net, pairs=None):
+        """learned warm start for `pairs` (all if None) from the network `net` (None: off)"""
+        arr = None
+        n = 0
+        if pairs is not None:
+            n = len(pairs)
+            arr = (ctypes.c_uint * max(n, 1))(*pairs)
+        self._learned_net = net          # keeps the network alive while the batch uses it
+        self._chk(self.lib.pdf_batch_set_learned(self.ctx, net.ctx if net is not None else None, arr, n))
+
+    def metrics(self):
+        n = len(self.rows)
+        e = np.empty(n, np.float32)
+        r = np.empty(n, np.float32)
+        self._chk(self.lib.pdf_batch_metrics(self.ctx, _fptr(e), _fptr(r)))
+        return e, r
+
+    def epe(self, ref):
+        out = np.empty(len(self.rows), np.float32)
+        self._chk(self.lib.pdf_batch_epe(self.ctx, ref.ctx, _fptr(out)))
+        return out
+
+    def resolve_plain(self, pairs):
+        n = len(pairs)
+        arr = (ctypes.c_uint * max(n, 1))(*pairs)
+        self._chk(self.lib.pdf_batch_resolve_plain(self.ctx, arr, n))
+
+    def fallback(self, energy_ref, residual_ref, tol=0.05):
+        """re-solves the pairs failing the rule; returns their flags (uint8 per pair)"""
+        n = len(self.rows)
+        e = np.ascontiguousarray(energy_ref, np.float32)
+        r = np.ascontiguousarray(residual_ref, np.float32)
+        flags = np.zeros(n, np.uint8)
+        cnt = ctypes.c_uint()
+        self._chk(self.lib.pdf_batch_fallback(self.ctx, _fptr(e), _fptr(r), tol,
+                                              flags.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)),
+                                              ctypes.byref(cnt)))
+        return flags
+
+    def learn_samples(self, h, w, x_ptr, t_ptr, m_ptr):
+        self._chk(self.lib.pdf_batch_learn_samples(self.ctx, h, w, ctypes.c_void_p(x_ptr), ctypes.c_void_p(t_ptr),
+                                                   ctypes.c_void_p(m_ptr)))

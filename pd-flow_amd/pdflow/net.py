@@ -27,6 +27,8 @@ def _sig(lib):
         "pdf_net_adam_step": ([vp, c_float], c_int),
         "pdf_net_train_step": ([vp, vp, vp, vp, c_uint, c_float, fp], c_int),
         "pdf_net_synchronize": ([vp], c_int),
+        "pdf_net_train_step_indexed": ([vp, vp, vp, vp, ctypes.c_size_t, ctypes.POINTER(c_uint), c_uint, c_float,
+                                        fp], c_int),
         "pdf_net_set_stream": ([vp, vp], c_int),
         "pdf_net_save": ([vp, ctypes.c_char_p], c_int),
         "pdf_net_load": ([vp, ctypes.c_char_p], c_int),
@@ -127,6 +129,16 @@ class Net:
         self._chk(self.lib.pdf_net_train_step(self.ctx, ctypes.c_void_p(x_ptr), ctypes.c_void_p(t_ptr),
                                               ctypes.c_void_p(m_ptr), batch, lr,
                                               ctypes.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def train_step_indexed(self, x_ptr, t_ptr, m_ptr, total, idx, lr, want_loss=False):
+        """one Adam step on samples idx of a device-resident data set"""
+        idx = np.ascontiguousarray(idx, np.uint32)
+        loss = ctypes.c_float(0.0)
+        self._chk(self.lib.pdf_net_train_step_indexed(
+            self.ctx, ctypes.c_void_p(x_ptr), ctypes.c_void_p(t_ptr), ctypes.c_void_p(m_ptr), total,
+            idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), idx.size, lr,
+            ctypes.byref(loss) if want_loss else None))
         return loss.value if want_loss else None
 
     def synchronize(self):

@@ -22,6 +22,8 @@ def _load():
         lib.pdf_synth_pair_float.argtypes = [ctypes.c_uint64, ctypes.c_uint, ctypes.c_uint, fp, fp, fp, fp, fp]
         lib.pdf_synth_static_float.argtypes = [ctypes.c_uint64, ctypes.c_uint, ctypes.c_uint, fp, fp]
         lib.pdf_synth_sequence_float.argtypes = [ctypes.c_uint64, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, fp, fp]
+        lib.pdf_synth_pairs_float.argtypes = [ctypes.c_uint64, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                              fp, fp, fp, fp, fp, ctypes.c_int]
         lib.pdf_synth_hash.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
         lib.pdf_synth_hash.restype = ctypes.c_uint64
         _lib = lib
@@ -63,6 +65,22 @@ def synth_pair_float(seed, rows, cols, with_gt=False):
     if with_gt:
         return (*arrs, gt.reshape(3, P))
     return tuple(arrs)
+
+
+def synth_pairs_float(seed0, n, rows, cols, threads=None, with_gt=False):
+    """n pairs (seeds seed0..seed0+n-1) as four (n, rows*cols) float arrays (plus the
+    (n, 3, rows*cols) ground-truth motion if asked), rendered on `threads` host threads
+    (default: up to 16)."""
+    P = rows * cols
+    arrs = [np.empty((n, P), np.float32) for _ in range(4)]
+    gt = np.empty((n, 3, P), np.float32) if with_gt else None
+    if threads is None:
+        threads = min(16, os.cpu_count() or 1)
+    rc = _load().pdf_synth_pairs_float(seed0, n, rows, cols, *[_p(a, ctypes.c_float) for a in arrs],
+                                       _p(gt, ctypes.c_float) if with_gt else None, threads)
+    if rc:
+        raise RuntimeError(f"pdf_synth_pairs_float failed ({rc})")
+    return (*arrs, gt) if with_gt else tuple(arrs)
 
 
 def synth_static_float(seed, rows, cols):

@@ -1,0 +1,203 @@
+"""GPU tests of the learned warm start inside the batched solver and of the per-pair quality
+reductions and fallback rule (include/pdflow_hip.h, pdf_batch_set_learned ... learn_samples)."""
+import numpy as np
+import pytest
+
+import metrics as mt
+
+pytestmark = pytest.mark.gpu
+
+ROWS = [240, 120, 240]          # QVGA (cam_mode 2), 120x160 (cam_mode 4), QVGA
+SEED = 9100
+
+
+@pytest.fixture(scope="module")
+def pdflow(gpu_available):
+    import pdflow
+    return pdflow
+
+
+def _fill(pdflow, b, rows, seed=SEED):
+    for i, r in enumerate(rows):
+        b.set_inputs(i, *pdflow.synth_pair_float(seed + i, r, r * 4 // 3))
+
+
+def _plain(pdflow, rows=ROWS, seed=SEED):
+    b = pdflow.Batch(rows)
+    _fill(pdflow, b, rows, seed)
+    b.solve(graph=True)
+    b.synchronize()
+    return b
+
+
+def _cut(pdflow, rows, factor=4):
+    """the reference schedule of each pair with every level's count divided by factor"""
+    return [[max(1, n // factor) for n in pdflow.num_max_iter(pdflow.ctf_levels_for_rows(r))] for r in rows]
+
+
+def _flows(b, n):
+    return [b.flow(i) for i in range(n)]
+
+
+def test_metrics_match_numpy(pdflow):
+    b = _plain(pdflow)
+    e, r = b.metrics()
+    p = pdflow.default_params(480, 1)
+    for i, rows in enumerate(ROWS):
+        cols = rows * 4 // 3
+        names = ["dct", "dcu", "dcv", "ddt", "ddu", "ddv", "mu_uv", "ri", "rj", "du_prev", "dv_prev",
+                 "du_new", "dv_new", "dw_new"]
+        f = {n: b.field(i, n) for n in names}
+        want_e = mt.energy(f, rows, cols, p["lambda_i"], p["lambda_d"])
+        I1, _, _ = b.pyramid(i, "colour_old", 0)
+        I2, _, _ = b.pyramid(i, "colour", 0)
+        Z1, _, _ = b.pyramid(i, "depth_old", 0)
+        want_r = mt.residual(I1, I2, Z1, b.field(i, "du_l"), b.field(i, "dv_l"), rows, cols)
+        assert abs(e[i] - want_e) <= 1e-4 * want_e, (i, e[i], want_e)
+        assert abs(r[i] - want_r) <= 1e-4 * want_r, (i, r[i], want_r)
+        assert want_r < 20.0          # a converged solve explains most of the intensity change
+
+
+def test_epe_matches_numpy(pdflow):
+    a = _plain(pdflow)
+    assert np.all(a.epe(a) == 0)
+    c = pdflow.Batch(ROWS, iters=_cut(pdflow, ROWS))
+    _fill(pdflow, c, ROWS)
+    c.solve(graph=False)
+    c.synchronize()
+    got = c.epe(a)
+    for i, rows in enumerate(ROWS):
+        Z1, _, _ = a.pyramid(i, "depth_old", 0)
+        want = mt.epe(c.flow(i), a.flow(i), Z1)
+        assert abs(got[i] - want) <= 1e-4 * want + 1e-9, (i, got[i], want)
+        assert want > 0
+
+
+def test_learn_samples(pdflow):
+    from pdflow.lib import DeviceBuffer
+    b = _plain(pdflow)
+    n, h, w = len(ROWS), 60, 80
+    bx, bt, bm = DeviceBuffer(n * h * w * 16), DeviceBuffer(n * h * w * 12), DeviceBuffer(n * h * w * 4)
+    b.learn_samples(h, w, bx.ptr.value, bt.ptr.value, bm.ptr.value)
+    x = bx.download(np.empty((n, h, w, 4), np.float32))
+    t = bt.download(np.empty((n, h, w, 3), np.float32))
+    m = bm.download(np.empty((n, h, w), np.float32))
+    assert np.all(m == 1)
+    for i, rows in enumerate(ROWS):
+        level = {240: 2, 120: 1}[rows]         # pyramid image of the 60x80 level
+        planes = [b.pyramid(i, k, level)[0].reshape(w, h).T for k in ("colour_old", "depth_old", "colour", "depth")]
+        want = np.stack([planes[0] / np.float32(255) - np.float32(0.5), planes[1] * np.float32(0.25),
+                         planes[2] / np.float32(255) - np.float32(0.5), planes[3] * np.float32(0.25)], axis=-1)
+        np.testing.assert_allclose(x[i], want, rtol=0, atol=1e-6)
+        s = rows // h
+        for c, name in enumerate(("du_l", "dv_l", "dw_l")):
+            fine = b.field(i, name).reshape(rows * 4 // 3, rows).T.astype(np.float64)
+            blk = fine.reshape(h, s, w, s).mean(axis=(1, 3))
+            want_t = blk / s if c < 2 else blk * 10
+            np.testing.assert_allclose(t[i, ..., c], want_t, rtol=1e-5, atol=1e-6)
+
+
+def _net(pdflow, n, seed=1, scale=None):
+    net = pdflow.Net(max_batch=n, h=60, w=80, seed=seed)
+    if scale is not None:
+        net.set_params(net.params() * np.float32(scale))
+    return net
+
+
+def test_learned_equals_explicit_warm_start(pdflow):
+    """network inside the solve == gather -> forward -> inject -> warm start done by hand"""
+    from pdflow.lib import DeviceBuffer
+    n = len(ROWS)
+    net = _net(pdflow, n)
+    sched = _cut(pdflow, ROWS)
+    lb = pdflow.Batch(ROWS, iters=sched)
+    _fill(pdflow, lb, ROWS)
+    lb.set_learned(net)
+    lb.solve(graph=False)
+    lb.synchronize()
+    got = _flows(lb, n)
+    # by hand: network inputs of a plain batch's pyramids, forward, then warm planes
+    pb = _plain(pdflow)
+    h, w = 60, 80
+    bx, bt, bm = DeviceBuffer(n * h * w * 16), DeviceBuffer(n * h * w * 12), DeviceBuffer(n * h * w * 4)
+    pb.learn_samples(h, w, bx.ptr.value, bt.ptr.value, bm.ptr.value)
+    x = bx.download(np.empty((n, h, w, 4), np.float32))
+    y = net.forward(x)
+    assert np.isfinite(y).all() and np.abs(y).max() > 0.1
+    wb = pdflow.Batch(ROWS, iters=sched)
+    _fill(pdflow, wb, ROWS)
+    for i in range(n):
+        ptr = wb.set_start_level(i, 2)           # the 60x80 level of 120- and 240-row pairs
+        planes = np.concatenate([y[i, ..., 0].T.ravel(), y[i, ..., 1].T.ravel(), (y[i, ..., 2] / np.float32(10)).T.ravel()])
+        rc = wb.lib.pdf_memcpy_h2d(ptr, np.ascontiguousarray(planes, np.float32).ctypes.data, planes.nbytes)
+        assert rc == 0
+    wb.solve(graph=False)
+    wb.synchronize()
+    want = _flows(wb, n)
+    for i in range(n):
+        for c in range(3):
+            assert np.array_equal(got[i][c], want[i][c]), (i, c)
+    # the captured graph replays the same solve
+    lb.solve(graph=True)
+    lb.synchronize()
+    lb.solve(graph=True)
+    lb.synchronize()
+    for i in range(n):
+        for c in range(3):
+            assert np.array_equal(lb.flow(i)[c], want[i][c]), (i, c)
+
+
+def test_resolve_plain_and_fallback(pdflow):
+    n = len(ROWS)
+    plain = _plain(pdflow)
+    pe, pr = plain.metrics()
+    want = _flows(plain, n)
+    lb = pdflow.Batch(ROWS, iters=_cut(pdflow, ROWS))
+    _fill(pdflow, lb, ROWS)
+    lb.set_learned(_net(pdflow, n))
+    lb.solve(graph=True)
+    lb.synchronize()
+    learned = _flows(lb, n)
+    # nothing exceeds an infinite reference: no fallback, flow unchanged
+    flags = lb.fallback(np.full(n, np.inf), np.full(n, np.inf))
+    assert not flags.any()
+    for i in range(n):
+        assert np.array_equal(lb.flow(i)[0], learned[i][0])
+    # references far below: every pair falls back and then equals its plain solve bit for bit
+    flags = lb.fallback(pe * 0.01, pr * 0.01)
+    assert flags.all()
+    for i in range(n):
+        for c in range(3):
+            assert np.array_equal(lb.flow(i)[c], want[i][c]), (i, c)
+    # a corrupted network output (weights scaled up 3x per layer) starts every pair far off:
+    # energy or residual end above the plain solver's and every pair falls back
+    bad = pdflow.Batch(ROWS, iters=_cut(pdflow, ROWS))
+    _fill(pdflow, bad, ROWS)
+    bad.set_learned(_net(pdflow, n, seed=3, scale=3.0))
+    bad.solve(graph=True)
+    bad.synchronize()
+    be, br = bad.metrics()
+    assert np.all((be > 1.05 * pe) | (br > 1.05 * pr))
+    flags = bad.fallback(pe, pr)
+    assert flags.all()
+    for i in range(n):
+        assert np.array_equal(bad.flow(i)[2], want[i][2])
+    # the next solve of the batch is the learned one again (plan state restored)
+    lb.solve(graph=True)
+    lb.synchronize()
+    for i in range(n):
+        assert np.array_equal(lb.flow(i)[1], learned[i][1])
+
+
+def test_set_learned_rejects_bad_pairs(pdflow):
+    net = _net(pdflow, 1)
+    b = pdflow.Batch([240, 240])
+    with pytest.raises(pdflow.PDFlowError):
+        b.set_learned(net)                      # two pairs, max_batch 1
+    with pytest.raises(pdflow.PDFlowError):
+        b.set_learned(net, [5])
+    small = pdflow.Batch([30])                  # 30x40: no 60x80 level
+    with pytest.raises(pdflow.PDFlowError):
+        small.set_learned(net)
+    b.set_learned(net, [1])
+    b.set_learned(None)
