@@ -1,11 +1,17 @@
 """Benchmark of the PD-Flow scene-flow path on one MI355X.
 
-Default workload: the plain PD-Flow drop-in path on one 480x640 (VGA) RGB-D pair per step --
-ingest both frames, build both Gaussian pyramids, solve 6 levels with the reference schedule
-(25, 40, 55, 70, 85, 100 iterations), flow left in HBM.  Inputs are seeded synthetic pairs already
-resident in HBM when the timed region starts.  One JSON line is printed, including the
-primal-dual iteration kernel's roofline (148 algorithmic bytes per pixel-iteration over its
-HIP-event-timed launches) and a CPU baseline.
+Default workload (--workload learned): 256 VGA (480x640) RGB-D pairs per step, batched with the
+learned warm start -- both pyramids of every pair, one network pass over their 60x80 levels,
+the solves from there with every level's iterations divided by 4, then the fallback rule (final
+energy or warped residual more than 5% above the plain solver's) and the plain re-solve of the
+pairs it rejects, all inside the timed region.  The line also carries the plain single-pair and
+plain batched rates of the same pairs, end-point errors against the plain solve and against the
+rendered ground truth, and the fallback fraction.
+
+--workload plain: the drop-in path on one VGA pair per step (reference schedule 25..100).
+Inputs are seeded synthetic pairs already resident in HBM when the timed region starts.  One
+JSON line is printed, including the primal-dual iteration launches' roofline (148 algorithmic
+bytes per pixel-iteration over their HIP-event-timed launches) and a CPU baseline.
 """
 import argparse
 import json
@@ -51,8 +57,8 @@ def cpu_baseline(rows, cam_mode, seed, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=480)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-graph", action="store_true")
@@ -61,10 +67,16 @@ def main():
     ap.add_argument("--tb-k", type=int, default=None,
                     help="iterations per launch of the temporally blocked kernel (1 = off)")
     ap.add_argument("--tb-min", type=int, default=None, help="smallest level (pixels) it runs on")
-    ap.add_argument("--workload", choices=("plain", "batched", "mixed", "stream"), default="plain",
+    ap.add_argument("--workload", choices=("plain", "batched", "mixed", "stream", "learned"), default="learned",
                     help="plain: one pair per step; batched: --batch-pairs pairs of --rows per "
-                         "step; mixed: 128 pairs (50%% 480x640, 30%% 240x320, 20%% 120x160)")
+                         "step; mixed: 128 pairs (50%% 480x640, 30%% 240x320, 20%% 120x160); "
+                         "learned: --batch-pairs pairs of --rows per step, network warm start at "
+                         "60x80, iterations / --factor, fallback rule")
     ap.add_argument("--batch-pairs", type=int, default=256)
+    ap.add_argument("--factor", type=int, default=4, help="learned: iteration cut per level")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "pd-flow_amd", "pdflow", "weights", "warmstart_qvga.bin"))
+    ap.add_argument("--fallback", choices=("energy_residual", "residual", "none"), default="energy_residual",
+                    help="learned: rule deciding which pairs the plain path re-solves (tolerance 5%%)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -84,6 +96,8 @@ def main():
         sys.exit(2)
     if args.workload == "stream":
         return run_stream(args)
+    if args.workload == "learned":
+        return run_learned(args)
     if args.workload != "plain":
         return run_batched(args)
     rows, cam_mode = args.rows, 480 // args.rows   # input frames at the solve resolution
@@ -281,6 +295,166 @@ def run_batched(args):
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "launches": launches, "bytes_per_pixel_iteration": BYTES_PER_PIXEL_ITER},
     }
+    print(json.dumps(out), flush=True)
+
+
+def plain_single_pair_rate(rows, seed, steps=20):
+    """pairs/s of the plain single-pair path (drop-in, hipGraph) on one pair of `rows`"""
+    from pdflow import Solver, synth_pair_float
+    from pdflow.lib import DeviceBuffer
+    cam = 480 // rows
+    planes = synth_pair_float(seed, 480 // cam, 640 // cam)
+    bufs = []
+    for a in planes:
+        b = DeviceBuffer(a.nbytes)
+        b.upload(a)
+        bufs.append(b)
+    s = Solver(rows, cam)
+    for _ in range(3):
+        s.solve_pair_device(bufs, graph=True)
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.solve_pair_device(bufs, graph=True)
+    s.synchronize()
+    return steps / (time.perf_counter() - t0)
+
+
+def run_learned(args):
+    """Batched + learned warm start: one step = one batched solve of all pairs (pyramids, one
+    network pass over their 60x80 levels, injection, solves with iterations / factor), then the
+    fallback rule and the plain re-solve of the pairs it rejects."""
+    from pdflow import Batch, Net, synth_pairs_float
+    from pdflow.train import cut_schedule
+    rows, n = args.rows, args.batch_pairs
+    cam = 480 // rows
+    if not os.path.exists(args.weights):
+        raise SystemExit(f"bench.py: network weights {args.weights} are missing")
+    log(f"bench: rendering {n} pairs of {480 // cam}x{640 // cam}")
+    *planes, gt = synth_pairs_float(args.seed, n, 480 // cam, 640 // cam, with_gt=True)
+    net = Net(max_batch=n, h=60, w=80)
+    net.load(args.weights)
+    cut = cut_schedule(rows, args.factor)
+
+    def make(iters=None):
+        b = Batch([rows] * n, iters=None if iters is None else [iters] * n)
+        for i in range(n):
+            b.set_inputs(i, *(p[i] for p in planes))
+        return b
+
+    # references of the fallback rule and of the EPE: the plain solve of the same pairs (untimed)
+    plain = make()
+    plain.solve(graph=True)
+    plain.synchronize()
+    pe, pr = plain.metrics()
+    learned = make(cut)
+    learned.set_learned(net)
+    ablation = make(cut)
+    inf = np.full(n, np.inf, np.float32)
+    e_ref = {"energy_residual": pe, "residual": inf, "none": inf}[args.fallback]
+    r_ref = {"energy_residual": pr, "residual": pr, "none": inf}[args.fallback]
+
+    def step():
+        learned.solve(graph=not args.no_graph)
+        return learned.fallback(e_ref, r_ref, 0.05)
+
+    for _ in range(max(1, args.warmup)):
+        flags = step()
+    learned.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        flags = step()
+    learned.synchronize()
+    el = time.perf_counter() - t0
+    pairs_s = n * args.steps / el
+    log(f"bench: {args.steps} learned steps in {el:.3f} s -> {pairs_s:.1f} pairs/s "
+        f"(fallback {int(flags.sum())}/{n})")
+    # the learned solve alone (no rule), and the plain batched solve, for the same pairs
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        learned.solve(graph=not args.no_graph)
+    learned.synchronize()
+    learned_only = n * args.steps / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps // 4)):
+        plain.solve(graph=True)
+    plain.synchronize()
+    plain_batched = n * max(1, args.steps // 4) / (time.perf_counter() - t0)
+    plain_single = plain_single_pair_rate(rows, args.seed)
+    log(f"bench: learned alone {learned_only:.1f}, plain batched {plain_batched:.1f}, "
+        f"plain single pair {plain_single:.1f} pairs/s")
+    # quality: EPE against the plain solve and against the rendered ground truth
+    def gt_epe(b):
+        out = []
+        for i in range(n):
+            ok = planes[1][i] > 0
+            d = np.sqrt(sum((f.astype(np.float64) - g) ** 2 for f, g in zip(b.flow(i), gt[i])))[ok]
+            out.append(d.mean())
+        return np.asarray(out)
+
+    learned.solve(graph=True)
+    learned.synchronize()
+    epe_l = learned.epe(plain)
+    le, lr_ = learned.metrics()
+    gt_learned = gt_epe(learned)
+    flags = learned.fallback(e_ref, r_ref, 0.05)
+    epe_f = learned.epe(plain)
+    ablation.solve(graph=True)
+    ablation.synchronize()
+    epe_a = ablation.epe(plain)
+    gt_plain, gt_final, gt_abl = gt_epe(plain), gt_epe(learned), gt_epe(ablation)
+
+    def stats(a):
+        return {"mean": round(float(np.mean(a)), 6), "median": round(float(np.median(a)), 6),
+                "p90": round(float(np.percentile(a, 90)), 6), "max": round(float(np.max(a)), 6)}
+
+    # roofline of the dominant kernel, the temporally blocked iteration kb_pd_tb<4>: HIP events
+    # around each of its launches on the batch's stream, over eager passes of the learned solve
+    learned.enable_kernel_timing(True, tb_only=True)
+    for _ in range(3):
+        learned.solve(graph=False)
+    launches, kms, pix = learned.kernel_timing()
+    learned.enable_kernel_timing(False)
+    achieved = BYTES_PER_PIXEL_ITER * pix / (kms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(pairs_s, 3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (network bf16 x bf16 -> f32)",
+        "data": f"synthetic (seeded RGB-D generator, seeds {args.seed}..{args.seed + n - 1})",
+        "config": {"workload": f"batched + learned warm start, {n} pairs of {rows}x{rows * 4 // 3}: "
+                               f"network at the 60x80 level, iterations {cut} (reference / {args.factor}), "
+                               f"fallback rule '{args.fallback}' (5%) with plain re-solves inside the timed "
+                               "region" + (", hipGraph" if not args.no_graph else ""),
+                   "pairs_per_step": n, "weights": os.path.relpath(args.weights, ROOT)},
+        "vs_plain": {"plain_single_pair_pairs_s": round(plain_single, 3),
+                     "plain_batched_pairs_s": round(plain_batched, 3),
+                     "learned_without_fallback_pairs_s": round(learned_only, 3),
+                     "speedup_vs_plain_single_pair": round(pairs_s / plain_single, 3),
+                     "speedup_without_fallback": round(learned_only / plain_single, 3)},
+        "quality_m": {"epe_vs_plain_learned": stats(epe_l), "epe_vs_plain_after_fallback": stats(epe_f),
+                      "epe_vs_plain_ablation": stats(epe_a), "epe_vs_truth_plain": stats(gt_plain),
+                      "epe_vs_truth_learned": stats(gt_learned), "epe_vs_truth_after_fallback": stats(gt_final),
+                      "epe_vs_truth_ablation": stats(gt_abl),
+                      "energy_ratio_learned": stats(le / pe), "residual_ratio_learned": stats(lr_ / pr),
+                      "fallback_fraction": round(float(flags.mean()), 4)},
+        "roofline": {"bound": "hbm", "kernel": "kb_pd_tb<4> (batched, 4 primal-dual iterations per launch)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "launches": launches,
+                     "avg_launch_us": round(kms / launches * 1e3, 3),
+                     "bytes_per_pixel_iteration": BYTES_PER_PIXEL_ITER},
+    }
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline ...")
+        out["cpu_baseline"] = cpu_baseline(rows, cam, args.seed, args.cpu_seconds)
+        out["cpu_baseline"]["sample"] += " (plain path: the CPU oracle has no learned path)"
     print(json.dumps(out), flush=True)
 
 

@@ -219,7 +219,7 @@ int pdf_batch_copy_flow(pdf_batch *b, unsigned pair, float *dx, float *dy, float
 /* number of kernel launches one batched solve makes */
 int pdf_batch_launch_count(const pdf_batch *b, unsigned *launches);
 /* HIP-event timing of the iteration launches, as pdf_enable_kernel_timing (solves run
- * eagerly while enabled) */
+ * eagerly while enabled); enable = 2 times only the temporally blocked launches */
 int pdf_batch_enable_kernel_timing(pdf_batch *b, int enable);
 int pdf_batch_kernel_timing(pdf_batch *b, uint64_t *launches, double *total_ms, uint64_t *pixel_iters);
 /* planes of pair's last solve on the device: a field of include/pdflow_fields.h (those the

@@ -108,6 +108,7 @@ struct pdf_batch {
     hipGraphExec_t graph = nullptr;
     // HIP-event timing of the iteration launches (eager solves only)
     bool timing = false;
+    bool timing_tb_only = false;     // time only the temporally blocked iteration launches
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     std::vector<uint64_t> event_pix;
     uint64_t timed_launches = 0, timed_pix = 0;
@@ -444,7 +445,7 @@ int run_plan(pdf_batch *b, const Plan &plan, char *dev_tables, bool timed)
             if (rc) return bfail(b, rc, std::string("network forward: ") + pdf_net_last_error(b->net));
             continue;
         }
-        const bool ev = timed && l.kind == Launch::PD;
+        const bool ev = timed && l.kind == Launch::PD && (!b->timing_tb_only || l.k > 1);
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (ev) {
             BHIP(b, hipEventCreate(&e0));
@@ -740,6 +741,7 @@ int pdf_batch_enable_kernel_timing(pdf_batch *b, int enable)
     int rc = pdf_batch_synchronize(b);
     if (rc) return rc;
     b->timing = enable != 0;
+    b->timing_tb_only = enable == 2;
     b->timed_launches = 0;
     b->timed_pix = 0;
     b->timed_ms = 0.0;

@@ -406,8 +406,8 @@ class Batch:
         self._chk(self.lib.pdf_batch_copy_flow(self.ctx, pair, *[_fptr(x) for x in out]))
         return tuple(out)
 
-    def enable_kernel_timing(self, on=True):
-        self._chk(self.lib.pdf_batch_enable_kernel_timing(self.ctx, 1 if on else 0))
+    def enable_kernel_timing(self, on=True, tb_only=False):
+        self._chk(self.lib.pdf_batch_enable_kernel_timing(self.ctx, (2 if tb_only else 1) if on else 0))
 
     def kernel_timing(self):
         n, ms, pix = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_uint64()

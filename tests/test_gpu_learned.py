@@ -201,3 +201,41 @@ def test_set_learned_rejects_bad_pairs(pdflow):
         small.set_learned(net)
     b.set_learned(net, [1])
     b.set_learned(None)
+
+
+def test_trained_network_beats_the_ablation(pdflow):
+    """The committed network (trained on other seeds) against the same 4x iteration cut without
+    it, on held-out QVGA pairs: end-point error against the rendered ground truth."""
+    import os
+    from pdflow.train import EVAL_SEED0, cut_schedule
+    weights = os.path.join(os.path.dirname(pdflow.__file__), "weights", "warmstart_qvga.bin")
+    n, rows = 32, 240
+    *planes, gt = pdflow.synth_pairs_float(EVAL_SEED0 + 5000, n, rows, 320, with_gt=True)
+    net = pdflow.Net(max_batch=n, h=60, w=80)
+    net.load(weights)
+    cut = [cut_schedule(rows, 4)] * n
+
+    def solve(iters, learned):
+        b = pdflow.Batch([rows] * n, iters=iters)
+        for i in range(n):
+            b.set_inputs(i, *(p[i] for p in planes))
+        if learned:
+            b.set_learned(net)
+        b.solve(graph=True)
+        b.synchronize()
+        return b
+
+    def gt_epe(b):
+        out = []
+        for i in range(n):
+            ok = planes[1][i] > 0
+            out.append(np.sqrt(sum((f.astype(np.float64) - g) ** 2 for f, g in zip(b.flow(i), gt[i])))[ok].mean())
+        return np.asarray(out)
+
+    plain, learned, ablation = solve(None, False), solve(cut, True), solve(cut, False)
+    e_plain, e_learned, e_abl = gt_epe(plain).mean(), gt_epe(learned).mean(), gt_epe(ablation).mean()
+    # the network's gain over the cut schedule alone, and accuracy within 2 mm of the plain solve's
+    assert e_learned < e_abl - 0.003, (e_learned, e_abl)
+    assert e_learned <= e_plain + 0.002, (e_learned, e_plain)
+    # closer to the plain solution too
+    assert learned.epe(plain).mean() < ablation.epe(plain).mean() + 0.005
