@@ -27,6 +27,11 @@ import numpy as np  # noqa: E402
 METRIC = "VGA scene-flow pairs/s: batched+learned-init vs plain, same EPE; HBM % peak"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PIXEL_ITER = 148     # 11 constant planes read + 13 state planes read and written, FP32
+# HBM bytes per kb_pd_tb<4> launch over the learned solve's launch mix at 256 VGA pairs, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of this workload, scaled by the counters'
+# readings for a plain copy of known size (fetch x 2.0, write x 1.0); the passes and the per-level
+# figures are in profiles/pd_iteration_batched_learned_vga256_hbm_traffic.txt
+TRAFFIC_TB4_LEARNED_VGA256 = 8.250e9
 
 
 def log(msg):
@@ -75,6 +80,9 @@ def main():
     ap.add_argument("--batch-pairs", type=int, default=256)
     ap.add_argument("--factor", type=int, default=4, help="learned: iteration cut per level")
     ap.add_argument("--weights", default=os.path.join(ROOT, "pd-flow_amd", "pdflow", "weights", "warmstart_qvga.bin"))
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first run a plain device copy of BYTES three times (the calibration point "
+                         "of HBM byte counters under rocprofv3 --pmc)")
     ap.add_argument("--fallback", choices=("energy_residual", "residual", "none"), default="energy_residual",
                     help="learned: rule deciding which pairs the plain path re-solves (tolerance 5%%)")
     args = ap.parse_args()
@@ -90,6 +98,17 @@ def main():
     if device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+
+    if args.copy_calibration:
+        from pdflow import hip_library
+        nb = args.copy_calibration // 16 * 16
+        src, dst = DeviceBuffer(nb), DeviceBuffer(nb)
+        for _ in range(3):
+            if hip_library().pdf_device_copy(dst.ptr, src.ptr, nb):
+                raise RuntimeError("pdf_device_copy failed")
+        log(f"bench: copy calibration, 3 copies of {nb} bytes")
+        src.free()
+        dst.free()
 
     if args.rows not in (15, 30, 60, 120, 240, 480):
         print("bench.py: --rows must be one of 15, 30, 60, 120, 240, 480", file=sys.stderr)
@@ -175,7 +194,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_pd_fused (fused gradient+dual+divergence+primal iteration)",
+            "kernel": "primal-dual iteration launches (k_pd_tb<4>, k_pd_level, k_pd_fused)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -447,7 +466,11 @@ def run_learned(args):
                       "fallback_fraction": round(float(flags.mean()), 4)},
         "roofline": {"bound": "hbm", "kernel": "kb_pd_tb<4> (batched, 4 primal-dual iterations per launch)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "launches": launches,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (TRAFFIC_TB4_LEARNED_VGA256
+                                 if (rows, n, args.factor) == (480, 256, 4) else None),
+                     "traffic_algorithmic_bytes": round(BYTES_PER_PIXEL_ITER * pix / launches),
+                     "launches": launches,
                      "avg_launch_us": round(kms / launches * 1e3, 3),
                      "bytes_per_pixel_iteration": BYTES_PER_PIXEL_ITER},
     }

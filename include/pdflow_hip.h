@@ -268,6 +268,10 @@ int pdf_device_alloc(void **ptr, size_t bytes);
 int pdf_device_free(void *ptr);
 int pdf_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int pdf_memcpy_d2h(void *dst, const void *src, size_t bytes);
+/* blocking device-to-device copy by a plain streaming kernel (bytes a multiple of 16, 16-byte
+ * aligned pointers): every byte read once and written once, the calibration point for HBM byte
+ * counters */
+int pdf_device_copy(void *dst, const void *src, size_t bytes);
 int pdf_device_count(int *count);
 
 #ifdef __cplusplus

@@ -814,4 +814,11 @@ int pdf_memcpy_d2h(void *dst, const void *src, size_t bytes)
     return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? PDF_OK : PDF_ERR_HIP;
 }
 
+int pdf_device_copy(void *dst, const void *src, size_t bytes)
+{
+    if (!dst || !src || bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16) return PDF_ERR_ARG;
+    if (pdf::launch_stream_copy(src, dst, bytes, nullptr) != hipSuccess) return PDF_ERR_HIP;
+    return hipDeviceSynchronize() == hipSuccess ? PDF_OK : PDF_ERR_HIP;
+}
+
 }  // extern "C"

@@ -116,5 +116,7 @@ unsigned batch_blocks_per_level(unsigned pixels);
 hipError_t launch_pd_batch(int k, const BatchTable &t, unsigned total_blocks, hipStream_t s);
 unsigned pd_blocks(int k, unsigned rows, unsigned cols);
 hipError_t launch_motion_field(const LevelArgs &a, hipStream_t s);
+// device-to-device streaming copy (bytes a multiple of 16, 16-byte aligned pointers)
+hipError_t launch_stream_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 
 }  // namespace pdf

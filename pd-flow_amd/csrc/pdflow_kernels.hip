@@ -10,6 +10,8 @@
 // iteration.  Nothing here uses MFMA: every stage is a memory-bound stencil.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pdflow_internal.hpp"
 #include "pdflow_pixel.hpp"
 
@@ -514,5 +516,21 @@ hipError_t launch_filter(const LevelArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 hipError_t launch_motion_field(const LevelArgs &a, hipStream_t s) { PDF_LAUNCH(k_motion_field, a.rows * a.cols, a, s); }
+
+// ------------------------------------------------------------------ plain copy
+// A streaming 16-byte copy of known size: the calibration point for HBM byte counters
+// (every byte read once, written once).
+__global__ void __launch_bounds__(256) k_stream_copy(const float4 *src, float4 *dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+hipError_t launch_stream_copy(const void *src, void *dst, size_t bytes, hipStream_t s)
+{
+    const size_t n = bytes / 16;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(k_stream_copy, dim3(blocks ? blocks : 1), dim3(256), 0, s, (const float4 *)src, (float4 *)dst, n);
+    return hipGetLastError();
+}
 
 }  // namespace pdf

@@ -119,6 +119,7 @@ def hip_library():
         "pdf_solve_warm": ([vp, up, c_uint, vp], c_int),
         "pdf_stream_frame_device": ([vp, vp, vp, up, c_int, ctypes.POINTER(c_int)], c_int),
         "pdf_batch_enable_kernel_timing": ([vp, c_int], c_int),
+        "pdf_device_copy": ([vp, vp, ctypes.c_size_t], c_int),
         "pdf_batch_field_device": ([vp, c_uint, c_int, ctypes.POINTER(vp)], c_int),
         "pdf_batch_pyramid_device": ([vp, c_uint, c_int, c_uint, ctypes.POINTER(vp), up_, up_], c_int),
         "pdf_batch_set_learned": ([vp, vp, up_, c_uint], c_int),
