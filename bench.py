@@ -133,7 +133,7 @@ def main():
     s = Solver(rows, cam_mode)
     lib, ctx = s.lib, s.ctx
     if args.tb_k is not None or args.tb_min is not None:
-        s.set_iteration_kernel(args.tb_k if args.tb_k is not None else 4,
+        s.set_iteration_kernel(args.tb_k if args.tb_k is not None else 6,
                                args.tb_min if args.tb_min is not None else 120 * 160)
     arr = (np.ctypeslib.ctypes.c_uint * len(iters))(*iters)
     use_graph = 0 if args.no_graph else 1
@@ -194,7 +194,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "primal-dual iteration launches (k_pd_tb<4>, k_pd_level, k_pd_fused)",
+            "kernel": "primal-dual iteration launches (k_pd_tb<K>, k_pd_level, k_pd_fused)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -97,9 +97,14 @@ int pdf_pd_iterations(pdf_ctx *ctx, unsigned n);
 
 /* Iteration-kernel selection (no reference counterpart; the reference launches four kernels
  * per iteration): levels of at least min_pixels pixels run k iterations per launch in the
- * temporally blocked kernel (k in {2,3,4,6,8}); k <= 1 disables it.  Results are identical
- * for every setting.  Default: k = 4 from 60x80 up. */
+ * temporally blocked kernel (k in {2,3,4,5,6,8}); k <= 1 disables it.  Results are identical
+ * for every setting.  Default: k = 6 from 60x80 up (batches: k = 4, pdf_batch_set_iteration_kernel). */
 int pdf_set_iteration_kernel(pdf_ctx *ctx, int k, unsigned min_pixels);
+/* Region shape of the temporally blocked kernel (no reference counterpart): 0 = 64 rows x 32
+ * columns per 1024-thread workgroup, 1 = 32 x 32 per 512-thread workgroup (default: two
+ * workgroups share a CU, so one's loads overlap the other's arithmetic).  Results are identical
+ * for every shape. */
+int pdf_set_tb_shape(pdf_ctx *ctx, int shape);
 
 /* Solve-path options (no reference counterpart).  fused_setup: pdf_solve runs each level's
  * set-up in four fused launches (default) instead of the reference's eleven bridges; the
@@ -194,6 +199,8 @@ const char *pdf_batch_last_error(const pdf_batch *b);   /* NULL b: last create f
 /* iteration kernels for the batch (as pdf_set_iteration_kernel; level_kernel as in
  * pdf_set_solve_options); invalidates the recorded plan and graph */
 int pdf_batch_set_iteration_kernel(pdf_batch *b, int k, unsigned min_pixels, int level_kernel);
+/* region shape of the temporally blocked kernel for the batch (as pdf_set_tb_shape) */
+int pdf_batch_set_tb_shape(pdf_batch *b, int shape);
 /* device pointers of pair's four input planes (the caller may fill them directly) */
 int pdf_batch_inputs_device(pdf_batch *b, unsigned pair, float **I1, float **Z1, float **I2,
                             float **Z2);

@@ -97,6 +97,7 @@ struct pdf_batch {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int tb_k = 4;
+    int tb_shape = pdf::kTBShape32x32;
     unsigned tb_min_pixels = 60 * 80;
     bool level_kernel = true;
     // plan
@@ -348,7 +349,7 @@ void build_plan(pdf_batch *b, const std::vector<unsigned> &who, bool fallback, P
                 if (n == 0) continue;
                 Pair &p = b->pairs[active[j]];
                 args.push_back(pd_args(b, p, n));
-                blocks.push_back(pdf::pd_blocks(k, p.rows_i, p.cols_i));
+                blocks.push_back(pdf::pd_blocks(k, b->tb_shape, p.rows_i, p.cols_i));
                 ids.push_back((unsigned)j);
             }
             if (args.empty()) return false;
@@ -455,7 +456,7 @@ int run_plan(pdf_batch *b, const Plan &plan, char *dev_tables, bool timed)
         hipError_t e;
         switch (l.kind) {
         case Launch::Setup: e = pdf::launch_batch(l.setup, t, l.blocks, l.flag, b->stream); break;
-        case Launch::PD: e = pdf::launch_pd_batch(l.k, t, l.blocks, b->stream); break;
+        case Launch::PD: e = pdf::launch_pd_batch(l.k, b->tb_shape, t, l.blocks, b->stream); break;
         default: e = pdf::launch_learn(l.learn, t, l.blocks, nullptr, b->stream); break;
         }
         if (e != hipSuccess) return bfail(b, PDF_ERR_HIP, std::string("batched launch: ") + hipGetErrorString(e));
@@ -645,6 +646,17 @@ int pdf_batch_set_iteration_kernel(pdf_batch *b, int k, unsigned min_pixels, int
     b->tb_k = k > 1 ? k : 0;
     b->tb_min_pixels = min_pixels;
     b->level_kernel = level_kernel != 0;
+    return PDF_OK;
+}
+
+int pdf_batch_set_tb_shape(pdf_batch *b, int shape)
+{
+    if (!b) return PDF_ERR_ARG;
+    if (!pdf::pd_tb_shape_supported(shape)) return bfail(b, PDF_ERR_ARG, "unsupported region shape");
+    if (shape != b->tb_shape) {
+        drop_plan(b);
+        b->tb_shape = shape;
+    }
     return PDF_OK;
 }
 

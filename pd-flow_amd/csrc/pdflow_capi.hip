@@ -77,7 +77,8 @@ struct pdf_ctx {
 
     // iteration kernel selection: levels of at least tb_min_pixels pixels run tb_k
     // iterations per launch (temporal blocking); 0 disables it
-    int tb_k = 4;
+    int tb_k = 6;
+    int tb_shape = pdf::kTBShape32x32;
     unsigned tb_min_pixels = 60 * 80;
     // levels of at most 2048 pixels run all their iterations in one workgroup
     bool level_kernel = true;
@@ -451,7 +452,7 @@ static int pd_launch(pdf_ctx *c, int k)
         HIP_TRY(c, hipEventRecord(e0, c->stream));
     }
     const pdf::PDArgs a = pd_args(c);
-    int rc = check_launch(c, k == 1 ? pdf::launch_pd_fused(a, c->stream) : pdf::launch_pd_tb(a, k, c->stream),
+    int rc = check_launch(c, k == 1 ? pdf::launch_pd_fused(a, c->stream) : pdf::launch_pd_tb(a, k, c->tb_shape, c->stream),
                           "fused primal-dual iteration");
     if (rc) return rc;
     if (c->timing) {
@@ -505,6 +506,16 @@ int pdf_set_iteration_kernel(pdf_ctx *c, int k, unsigned min_pixels)
     if (k > 1 && !pdf::pd_tb_supported(k)) return fail(c, PDF_ERR_ARG, "unsupported iterations per launch");
     c->tb_k = k > 1 ? k : 0;
     c->tb_min_pixels = min_pixels;
+    for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
+    c->graphs.clear();
+    return PDF_OK;
+}
+
+int pdf_set_tb_shape(pdf_ctx *c, int shape)
+{
+    if (!c) return PDF_ERR_ARG;
+    if (!pdf::pd_tb_shape_supported(shape)) return fail(c, PDF_ERR_ARG, "unsupported region shape");
+    c->tb_shape = shape;
     for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
     c->graphs.clear();
     return PDF_OK;

@@ -65,10 +65,20 @@ struct BatchItem {
     unsigned item, block;
 };
 
-// the item that owns this block: largest i with first[i] <= blockIdx.x (binary search)
-__device__ __forceinline__ BatchItem batch_item(const BatchTable &t)
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2): block b runs
+// on XCD b % 8.  xcd_swizzle maps dispatch order to a tile order in which every XCD walks one
+// contiguous run of tiles, so neighbouring tiles (whose regions share halo rows and cache
+// lines) meet in the same L2.  A bijection on [0, n).
+constexpr unsigned kNumXCD = 8;
+__device__ __forceinline__ unsigned xcd_swizzle(unsigned b, unsigned n)
 {
-    const unsigned b = blockIdx.x;
+    const unsigned q = n / kNumXCD, r = n % kNumXCD, x = b % kNumXCD, i = b / kNumXCD;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// the item that owns block b: largest i with first[i] <= b (binary search)
+__device__ __forceinline__ BatchItem batch_item(const BatchTable &t, unsigned b)
+{
     unsigned lo = 0, hi = t.n;            // invariant: first[lo] <= b < first[hi]
     while (hi - lo > 1) {
         const unsigned mid = (lo + hi) / 2;
@@ -77,6 +87,8 @@ __device__ __forceinline__ BatchItem batch_item(const BatchTable &t)
     }
     return {lo, b - t.first[lo]};
 }
+
+__device__ __forceinline__ BatchItem batch_item(const BatchTable &t) { return batch_item(t, blockIdx.x); }
 
 enum class BatchKernel { LevelInit, RijGradients, WarpMu, FilterMotion, Pyramid, WarmInit };
 
@@ -97,7 +109,10 @@ hipError_t launch_divergence(const LevelArgs &a, hipStream_t s);
 hipError_t launch_primal(const LevelArgs &a, hipStream_t s);
 hipError_t launch_pd_fused(const PDArgs &a, hipStream_t s);
 // k fused iterations in one launch (temporal blocking); k must satisfy pd_tb_supported
-hipError_t launch_pd_tb(const PDArgs &a, int k, hipStream_t s);
+hipError_t launch_pd_tb(const PDArgs &a, int k, int shape, hipStream_t s);
+// region shapes of the temporally blocked kernel (rows x columns per workgroup)
+enum { kTBShape64x32 = 0, kTBShape32x32 = 1 };
+bool pd_tb_shape_supported(int shape);
 // all n iterations of a level of at most 2048 pixels in one single-workgroup launch
 hipError_t launch_pd_level(const PDArgs &a, unsigned n, hipStream_t s);
 bool pd_level_supported(unsigned pixels);
@@ -113,8 +128,8 @@ hipError_t launch_pyramid2(const Pyr2Args &a, hipStream_t s);
 hipError_t launch_batch(BatchKernel k, const BatchTable &t, unsigned total_blocks, int flag, hipStream_t s);
 unsigned batch_blocks_per_level(unsigned pixels);
 // k = 0: whole-level kernel (iters from PDArgs), 1: one iteration, >1: temporal blocking
-hipError_t launch_pd_batch(int k, const BatchTable &t, unsigned total_blocks, hipStream_t s);
-unsigned pd_blocks(int k, unsigned rows, unsigned cols);
+hipError_t launch_pd_batch(int k, int shape, const BatchTable &t, unsigned total_blocks, hipStream_t s);
+unsigned pd_blocks(int k, int shape, unsigned rows, unsigned cols);
 hipError_t launch_motion_field(const LevelArgs &a, hipStream_t s);
 // device-to-device streaming copy (bytes a multiple of 16, 16-byte aligned pointers)
 hipError_t launch_stream_copy(const void *src, void *dst, size_t bytes, hipStream_t s);

@@ -295,46 +295,53 @@ __device__ __forceinline__ void primal_px(Px &q, int rows, int cols, float riu, 
 }
 
 // ------------------------------------------------------------------ K iterations per launch
-constexpr int BV = 64, BU = 32;        // region: one wave64 per column pair, 16 waves
-constexpr int BT = 1024;               // threads per workgroup, two pixels (columns) each
+// Region shapes (rows RV x columns RU, RV*RU/2 threads, two pixels per thread in adjacent
+// columns): lanes of a wave run down RV consecutive rows of 64/RV column pairs.
+//   shape 0: 64 x 32, 1024 threads (one workgroup per CU: 88 KB LDS, ~118 VGPRs)
+//   shape 1: 32 x 32,  512 threads (two workgroups per CU, so one's loads overlap the other's
+//            arithmetic; more halo per output pixel)
+constexpr int BV = 64, BU = 32;        // shape 0, also the whole-level kernel's thread layout
+constexpr int BT = 1024;
 
 // applies f(pixel, j) to a thread's two pixels (named variables, so they stay in registers)
 #define PD_BOTH(f) do { f(x0, 0); f(x1, 1); } while (0)
 
-template <int K>
+template <int K, int RV, int RU>
 __device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned by)
 {
-    constexpr int TVk = BV - 2 * K, TUk = BU - 2 * K;
-    __shared__ float sU[BU * BV], sV[BU * BV], sW[BU * BV];
-    __shared__ float sP[6][BU * BV];
-    __shared__ float sRi[BU * BV], sRj[BU * BV];
+    constexpr int TVk = RV - 2 * K, TUk = RU - 2 * K;
+    static_assert(64 % RV == 0 && TVk > 0 && TUk > 0, "region shape");
+    __shared__ float sU[RU * RV], sV[RU * RV], sW[RU * RV];
+    __shared__ float sP[6][RU * RV];
+    __shared__ float sRi[RU * RV], sRj[RU * RV];
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rv0 = (int)bx * TVk - K, ru0 = (int)by * TUk - K;  // region origin
     const int rows = (int)a.rows, cols = (int)a.cols;
     const float li = a.lambda_i, ld = a.lambda_d;
-    const int lv = lane;
+    const int lv = lane % RV;
+    const int slot = wave * (64 / RV) + lane / RV;               // this thread's column pair
 
     Px x0, x1;
     auto load = [&](Px &q, int j) __attribute__((always_inline)) {
-        const int lu = 2 * wave + j;
+        const int lu = 2 * slot + j;
         const int gv = rv0 + lv, gu = ru0 + lu;
         load_px(q, a, gv, gu, gv >= 0 && gv < rows && gu >= 0 && gu < cols);
-        sRi[lu * BV + lv] = q.ri;
-        sRj[lu * BV + lv] = q.rj;
+        sRi[lu * RV + lv] = q.ri;
+        sRj[lu * RV + lv] = q.rj;
     };
     PD_BOTH(load);
     __syncthreads();
     // neighbours outside the region only feed pixels of the discarded band, so their
     // indices are clamped into the region
     auto derive = [&](Px &q, int j) __attribute__((always_inline)) {
-        const int lu = 2 * wave + j, idx = lu * BV + lv;
-        derive_px(q, sRi[lu > 0 ? idx - BV : idx], sRj[lv > 0 ? idx - 1 : idx], li, ld);
+        const int lu = 2 * slot + j, idx = lu * RV + lv;
+        derive_px(q, sRi[lu > 0 ? idx - RV : idx], sRj[lv > 0 ? idx - 1 : idx], li, ld);
     };
     PD_BOTH(derive);
 
     auto put_bar = [&](Px &q, int j) __attribute__((always_inline)) {
-        const int idx = (2 * wave + j) * BV + lv;
+        const int idx = (2 * slot + j) * RV + lv;
         sU[idx] = q.ua + q.up;
         sV[idx] = q.va + q.vp;
         sW[idx] = q.wa;
@@ -342,17 +349,17 @@ __device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned b
     // pixels outside the image skip the arithmetic: no pixel inside reads them
     auto dual = [&](Px &q, int j) __attribute__((always_inline)) {
         if (!q.on) return;
-        const int lu = 2 * wave + j, idx = lu * BV + lv;
-        const int iu = lu + 1 < BU ? idx + BV : idx;
-        const int iv = lv + 1 < BV ? idx + 1 : idx;
+        const int lu = 2 * slot + j, idx = lu * RV + lv;
+        const int iu = lu + 1 < RU ? idx + RV : idx;
+        const int iv = lv + 1 < RV ? idx + 1 : idx;
         dual_px(q, rows, cols, sU[iu], sV[iu], sW[iu], sU[iv], sV[iv], sW[iv]);
 #pragma unroll
         for (int k = 0; k < 6; k++) sP[k][idx] = q.p[k];
     };
     auto primal = [&](Px &q, int j) __attribute__((always_inline)) {
         if (!q.on) return;
-        const int lu = 2 * wave + j, idx = lu * BV + lv;
-        const int ju = lu > 0 ? idx - BV : idx;
+        const int lu = 2 * slot + j, idx = lu * RV + lv;
+        const int ju = lu > 0 ? idx - RV : idx;
         const int jv = lv > 0 ? idx - 1 : idx;
         primal_px(q, rows, cols, sRi[ju], sP[0][ju], sP[2][ju], sP[4][ju], sRj[jv], sP[1][jv],
                   sP[3][jv], sP[5][jv], li, ld);
@@ -367,7 +374,7 @@ __device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned b
 
     // write back the inner tile
     auto store = [&](Px &q, int j) __attribute__((always_inline)) {
-        const int lu = 2 * wave + j;
+        const int lu = 2 * slot + j;
         if (q.on && lv >= K && lv < K + TVk && lu >= K && lu < K + TUk) store_px(q, a);
     };
     PD_BOTH(store);
@@ -448,8 +455,13 @@ __device__ __forceinline__ void level_body(const PDArgs &a, int n)
 
 // ------------------------------------------------------------------ entry points
 __global__ void __launch_bounds__(TV * TU) k_pd_fused(PDArgs a) { pd_fused_body(a, blockIdx.x, blockIdx.y); }
-template <int K>
-__global__ void __launch_bounds__(BT) k_pd_tb(PDArgs a) { tb_body<K>(a, blockIdx.x, blockIdx.y); }
+template <int K, int RV, int RU>
+__global__ void __launch_bounds__(RV * RU / 2) k_pd_tb(PDArgs a)
+{
+    // tiles numbered v-major, visited in XCD-contiguous runs
+    const unsigned t = xcd_swizzle(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    tb_body<K, RV, RU>(a, t % gridDim.x, t / gridDim.x);
+}
 __global__ void __launch_bounds__(BT) k_pd_level(PDArgs a, int n) { level_body(a, n); }
 
 // batched: every table entry is one pair's PDArgs; tiles of a pair are numbered v-major
@@ -460,13 +472,13 @@ __global__ void __launch_bounds__(TV * TU) kb_pd_fused(BatchTable t)
     const unsigned tiles_v = (a.rows + TV - 1) / TV;
     pd_fused_body(a, it.block % tiles_v, it.block / tiles_v);
 }
-template <int K>
-__global__ void __launch_bounds__(BT) kb_pd_tb(BatchTable t)
+template <int K, int RV, int RU>
+__global__ void __launch_bounds__(RV * RU / 2) kb_pd_tb(BatchTable t)
 {
-    const BatchItem it = batch_item(t);
+    const BatchItem it = batch_item(t, xcd_swizzle(blockIdx.x, gridDim.x));
     const PDArgs &a = static_cast<const PDArgs *>(t.args)[it.item];
-    const unsigned tiles_v = (a.rows + (BV - 2 * K) - 1) / (BV - 2 * K);
-    tb_body<K>(a, it.block % tiles_v, it.block / tiles_v);
+    const unsigned tiles_v = (a.rows + (RV - 2 * K) - 1) / (RV - 2 * K);
+    tb_body<K, RV, RU>(a, it.block % tiles_v, it.block / tiles_v);
 }
 __global__ void __launch_bounds__(BT) kb_pd_level(BatchTable t)
 {
@@ -495,26 +507,45 @@ hipError_t launch_pd_level(const PDArgs &a, unsigned n, hipStream_t s)
 
 bool pd_level_supported(unsigned pixels) { return pixels <= (unsigned)kLevelMaxPixels; }
 
-bool pd_tb_supported(int k) { return k == 2 || k == 3 || k == 4 || k == 6 || k == 8; }
+bool pd_tb_supported(int k) { return k == 2 || k == 3 || k == 4 || k == 5 || k == 6 || k == 8; }
+bool pd_tb_shape_supported(int shape) { return shape == kTBShape64x32 || shape == kTBShape32x32; }
 
-template <int K>
+template <int RV, int RU>
+static void tb_region(int k, unsigned rows, unsigned cols, unsigned *tv, unsigned *tu)
+{
+    *tv = (rows + (RV - 2 * k) - 1) / (RV - 2 * k);
+    *tu = (cols + (RU - 2 * k) - 1) / (RU - 2 * k);
+}
+
+template <int K, int RV, int RU>
 static hipError_t launch_tb(const PDArgs &a, hipStream_t s)
 {
-    constexpr int TVk = BV - 2 * K, TUk = BU - 2 * K;
-    const dim3 grid((a.rows + TVk - 1) / TVk, (a.cols + TUk - 1) / TUk);
-    hipLaunchKernelGGL(k_pd_tb<K>, grid, dim3(BT), 0, s, a);
+    unsigned tv, tu;
+    tb_region<RV, RU>(K, a.rows, a.cols, &tv, &tu);
+    hipLaunchKernelGGL((k_pd_tb<K, RV, RU>), dim3(tv, tu), dim3(RV * RU / 2), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_pd_tb(const PDArgs &a, int k, hipStream_t s)
+template <int RV, int RU>
+static hipError_t launch_tb_shape(const PDArgs &a, int k, hipStream_t s)
+{
+    switch (k) {
+    case 2: return launch_tb<2, RV, RU>(a, s);
+    case 3: return launch_tb<3, RV, RU>(a, s);
+    case 4: return launch_tb<4, RV, RU>(a, s);
+    case 5: return launch_tb<5, RV, RU>(a, s);
+    case 6: return launch_tb<6, RV, RU>(a, s);
+    case 8: return launch_tb<8, RV, RU>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_pd_tb(const PDArgs &a, int k, int shape, hipStream_t s)
 {
     if (a.rows == 0 || a.cols == 0) return hipSuccess;
-    switch (k) {
-    case 2: return launch_tb<2>(a, s);
-    case 3: return launch_tb<3>(a, s);
-    case 4: return launch_tb<4>(a, s);
-    case 6: return launch_tb<6>(a, s);
-    case 8: return launch_tb<8>(a, s);
+    switch (shape) {
+    case kTBShape64x32: return launch_tb_shape<64, 32>(a, k, s);
+    case kTBShape32x32: return launch_tb_shape<32, 32>(a, k, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -523,29 +554,46 @@ hipError_t launch_pd_tb(const PDArgs &a, int k, hipStream_t s)
 
 namespace pdf {
 
-unsigned pd_blocks(int k, unsigned rows, unsigned cols)
+unsigned pd_blocks(int k, int shape, unsigned rows, unsigned cols)
 {
     if (k <= 0) return 1;                                   // whole-level kernel
     if (k == 1) return ((rows + TV - 1) / TV) * ((cols + TU - 1) / TU);
-    const unsigned tv = BV - 2 * k, tu = BU - 2 * k;
-    return ((rows + tv - 1) / tv) * ((cols + tu - 1) / tu);
+    unsigned tv, tu;
+    if (shape == kTBShape32x32) tb_region<32, 32>(k, rows, cols, &tv, &tu);
+    else tb_region<64, 32>(k, rows, cols, &tv, &tu);
+    return tv * tu;
 }
 
-hipError_t launch_pd_batch(int k, const BatchTable &t, unsigned total_blocks, hipStream_t s)
+template <int RV, int RU>
+static hipError_t launch_tb_batch(int k, const BatchTable &t, const dim3 &grid, hipStream_t s)
+{
+    const dim3 block(RV * RU / 2);
+    switch (k) {
+    case 2: hipLaunchKernelGGL((kb_pd_tb<2, RV, RU>), grid, block, 0, s, t); break;
+    case 3: hipLaunchKernelGGL((kb_pd_tb<3, RV, RU>), grid, block, 0, s, t); break;
+    case 4: hipLaunchKernelGGL((kb_pd_tb<4, RV, RU>), grid, block, 0, s, t); break;
+    case 5: hipLaunchKernelGGL((kb_pd_tb<5, RV, RU>), grid, block, 0, s, t); break;
+    case 6: hipLaunchKernelGGL((kb_pd_tb<6, RV, RU>), grid, block, 0, s, t); break;
+    case 8: hipLaunchKernelGGL((kb_pd_tb<8, RV, RU>), grid, block, 0, s, t); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pd_batch(int k, int shape, const BatchTable &t, unsigned total_blocks, hipStream_t s)
 {
     if (total_blocks == 0 || t.n == 0) return hipSuccess;
     const dim3 grid(total_blocks);
     switch (k) {
-    case 0: hipLaunchKernelGGL(kb_pd_level, grid, dim3(BT), 0, s, t); break;
-    case 1: hipLaunchKernelGGL(kb_pd_fused, grid, dim3(TV * TU), 0, s, t); break;
-    case 2: hipLaunchKernelGGL(kb_pd_tb<2>, grid, dim3(BT), 0, s, t); break;
-    case 3: hipLaunchKernelGGL(kb_pd_tb<3>, grid, dim3(BT), 0, s, t); break;
-    case 4: hipLaunchKernelGGL(kb_pd_tb<4>, grid, dim3(BT), 0, s, t); break;
-    case 6: hipLaunchKernelGGL(kb_pd_tb<6>, grid, dim3(BT), 0, s, t); break;
-    case 8: hipLaunchKernelGGL(kb_pd_tb<8>, grid, dim3(BT), 0, s, t); break;
-    default: return hipErrorInvalidValue;
+    case 0: hipLaunchKernelGGL(kb_pd_level, grid, dim3(BT), 0, s, t); return hipGetLastError();
+    case 1: hipLaunchKernelGGL(kb_pd_fused, grid, dim3(TV * TU), 0, s, t); return hipGetLastError();
+    default:
+        switch (shape) {
+        case kTBShape64x32: return launch_tb_batch<64, 32>(k, t, grid, s);
+        case kTBShape32x32: return launch_tb_batch<32, 32>(k, t, grid, s);
+        default: return hipErrorInvalidValue;
+        }
     }
-    return hipGetLastError();
 }
 
 }  // namespace pdf

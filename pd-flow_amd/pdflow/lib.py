@@ -120,6 +120,8 @@ def hip_library():
         "pdf_stream_frame_device": ([vp, vp, vp, up, c_int, ctypes.POINTER(c_int)], c_int),
         "pdf_batch_enable_kernel_timing": ([vp, c_int], c_int),
         "pdf_device_copy": ([vp, vp, ctypes.c_size_t], c_int),
+        "pdf_set_tb_shape": ([vp, c_int], c_int),
+        "pdf_batch_set_tb_shape": ([vp, c_int], c_int),
         "pdf_batch_field_device": ([vp, c_uint, c_int, ctypes.POINTER(vp)], c_int),
         "pdf_batch_pyramid_device": ([vp, c_uint, c_int, c_uint, ctypes.POINTER(vp), up_, up_], c_int),
         "pdf_batch_set_learned": ([vp, vp, up_, c_uint], c_int),
@@ -259,6 +261,10 @@ class Solver:
         """k iterations per launch (temporal blocking) on levels of >= min_pixels pixels."""
         self._chk(self.lib.pdf_set_iteration_kernel(self.ctx, k, min_pixels))
 
+    def set_tb_shape(self, shape):
+        """region shape of the temporally blocked kernel: 0 = 64x32 (1024 threads), 1 = 32x32"""
+        self._chk(self.lib.pdf_set_tb_shape(self.ctx, shape))
+
     def set_solve_options(self, fused_setup=True, level_kernel=True):
         self._chk(self.lib.pdf_set_solve_options(self.ctx, int(fused_setup), int(level_kernel)))
 
@@ -389,6 +395,9 @@ class Batch:
 
     def set_iteration_kernel(self, k, min_pixels=0, level_kernel=True):
         self._chk(self.lib.pdf_batch_set_iteration_kernel(self.ctx, k, min_pixels, int(level_kernel)))
+
+    def set_tb_shape(self, shape):
+        self._chk(self.lib.pdf_batch_set_tb_shape(self.ctx, shape))
 
     def set_inputs(self, pair, I1, Z1, I2, Z2):
         a = [np.ascontiguousarray(x, np.float32) for x in (I1, Z1, I2, Z2)]

@@ -21,11 +21,12 @@ def single(rows, seed, iters=None):
     return (I1, Z1, I2, Z2), g.solve_pair(I1, Z1, I2, Z2, iters=iters)
 
 
-@pytest.mark.parametrize("k,graph", [(4, True), (2, False), (0, True)])
-def test_mixed_batch_matches_single_pairs(gpu_available, k, graph):
+@pytest.mark.parametrize("k,graph,shape", [(4, True, 0), (2, False, 0), (0, True, 0), (4, True, 1), (3, False, 1)])
+def test_mixed_batch_matches_single_pairs(gpu_available, k, graph, shape):
     rows = [480, 240, 120, 60, 240, 30, 120, 15, 480]
-    say(f"mixed batch of {len(rows)} pairs, k={k}, graph={graph}")
+    say(f"mixed batch of {len(rows)} pairs, k={k}, graph={graph}, shape={shape}")
     b = Batch(rows)
+    b.set_tb_shape(shape)
     if k:
         b.set_iteration_kernel(k, 4800)
     else:

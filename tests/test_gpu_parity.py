@@ -205,13 +205,14 @@ def test_nan_and_empty_depth_inputs_do_not_break_the_solve(gpu_available):
         assert_same(x, y, n)
 
 
-@pytest.mark.parametrize("k", [2, 3, 4, 6, 8])
-def test_temporal_blocking_matches_oracle(gpu_available, k):
-    """k iterations per launch on every level (min_pixels=0), including remainders, against
-    the oracle's one-stage-at-a-time iterations: bit-identical expected."""
-    say(f"temporal blocking k={k}")
+@pytest.mark.parametrize("k,shape", [(2, 0), (3, 0), (4, 0), (6, 0), (8, 0), (2, 1), (4, 1), (8, 1)])
+def test_temporal_blocking_matches_oracle(gpu_available, k, shape):
+    """k iterations per launch on every level (min_pixels=0), including remainders, in both
+    region shapes, against the oracle's one-stage-at-a-time iterations: bit-identical expected."""
+    say(f"temporal blocking k={k} shape={shape}")
     g, o, _ = both(240, 2, seed=31 + k)
     g.set_iteration_kernel(k, 0)
+    g.set_tb_shape(shape)
     for (i, rows_i, cols_i, limg, iters) in level_table(240, 2):
         for s in (g, o):
             s.allocate_level(rows_i, cols_i, i, limg)
