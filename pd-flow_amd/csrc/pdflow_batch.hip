@@ -337,46 +337,42 @@ void build_plan(pdf_batch *b, const std::vector<unsigned> &who, bool fallback, P
         setup(pdf::BatchKernel::RijGradients, 0);
         setup(pdf::BatchKernel::WarpMu, 0);
 
-        // iterations: remaining count per active pair
-        std::vector<unsigned> rem;
-        for (unsigned i : active) rem.push_back(iters_of(b->pairs[i], s));
-        auto pd_round = [&](int k, auto take) {
+        // iterations: each active pair's launches (iterations per launch, 0 = whole level),
+        // consumed in rounds; one launch per round and launch size covers every pair whose
+        // next launch has that size
+        const unsigned P = active.empty() ? 0 : b->pairs[active[0]].rows_i * b->pairs[active[0]].cols_i;
+        const bool whole = b->level_kernel && pdf::pd_level_supported(P);
+        const int K = (b->tb_k > 1 && P >= b->tb_min_pixels) ? b->tb_k : 1;
+        std::vector<std::vector<int>> todo(active.size());
+        for (size_t j = 0; j < active.size(); j++) {
+            const unsigned n = iters_of(b->pairs[active[j]], s);
+            if (whole) { if (n) todo[j].push_back(0); }
+            else todo[j] = pdf::tb_chunks(n, K);
+        }
+        std::vector<size_t> pos(active.size(), 0);
+        for (;;) {
+            int k = -1;
+            for (size_t j = 0; j < active.size() && k < 0; j++)
+                if (pos[j] < todo[j].size()) k = todo[j][pos[j]];
+            if (k < 0) break;
             std::vector<pdf::PDArgs> args;
             std::vector<unsigned> blocks;
-            std::vector<unsigned> ids;
+            uint64_t pix = 0;
             for (size_t j = 0; j < active.size(); j++) {
-                const unsigned n = take(rem[j]);
-                if (n == 0) continue;
+                if (pos[j] >= todo[j].size() || todo[j][pos[j]] != k) continue;
                 Pair &p = b->pairs[active[j]];
+                const unsigned n = k == 0 ? iters_of(p, s) : (unsigned)k;
                 args.push_back(pd_args(b, p, n));
                 blocks.push_back(pdf::pd_blocks(k, b->tb_shape, p.rows_i, p.cols_i));
-                ids.push_back((unsigned)j);
+                pix += (uint64_t)p.rows_i * p.cols_i * n;
+                pos[j]++;
+                swap_state(p);
             }
-            if (args.empty()) return false;
-            uint64_t pix = 0;
-            for (size_t q = 0; q < args.size(); q++) pix += (uint64_t)args[q].rows * args[q].cols * args[q].iters;
             Launch l{};
             l.kind = Launch::PD;
             l.k = k;
             l.pixel_iters = pix;
             add_launch(plan, l, args, blocks);
-            for (unsigned j : ids) {
-                rem[j] -= (unsigned)(k == 0 ? rem[j] : (k == 1 ? 1 : k));
-                swap_state(b->pairs[active[j]]);
-            }
-            return true;
-        };
-        const unsigned P = active.empty() ? 0 : b->pairs[active[0]].rows_i * b->pairs[active[0]].cols_i;
-        if (b->level_kernel && pdf::pd_level_supported(P)) {
-            pd_round(0, [](unsigned r) { return r; });
-        } else {
-            if (b->tb_k > 1 && P >= b->tb_min_pixels) {
-                const unsigned K = (unsigned)b->tb_k;
-                while (pd_round(b->tb_k, [K](unsigned r) { return r >= K ? K : 0u; })) {
-                }
-            }
-            while (pd_round(1, [](unsigned r) { return r > 0 ? 1u : 0u; })) {
-            }
         }
         setup(pdf::BatchKernel::FilterMotion, 0);
     }

@@ -491,12 +491,8 @@ int pdf_pd_iterations(pdf_ctx *c, unsigned n)
     if (n == 0) return PDF_OK;
     if (c->level_kernel && pdf::pd_level_supported(c->rows_i * c->cols_i)) return pd_launch_level(c, n);
     const bool tb = c->tb_k > 1 && (size_t)c->rows_i * c->cols_i >= c->tb_min_pixels;
-    unsigned done = 0;
-    if (tb)
-        for (; done + (unsigned)c->tb_k <= n; done += c->tb_k)
-            if ((rc = pd_launch(c, c->tb_k))) return rc;
-    for (; done < n; done++)
-        if ((rc = pd_launch(c, 1))) return rc;
+    for (int k : pdf::tb_chunks(n, tb ? c->tb_k : 1))
+        if ((rc = pd_launch(c, k))) return rc;
     return PDF_OK;
 }
 

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "../../include/pdflow_fields.h"
 
 namespace pdf {
@@ -95,6 +97,39 @@ enum class BatchKernel { LevelInit, RijGradients, WarpMu, FilterMotion, Pyramid,
 // Iterations per launch supported by the temporally blocked kernel.
 constexpr int kTBMaxK = 8;
 bool pd_tb_supported(int k);
+
+// How n iterations of a level are split into launches of the temporally blocked kernel with k
+// iterations each (k = 1: the single-iteration kernel): launches of K, with the remainder (a
+// remainder of 1 joined to the last K) as one supported size or two balanced ones (7 = 4 + 3),
+// so that no level of 2 or more iterations ends with single-iteration launches, which cost
+// about twice a blocked iteration.
+inline std::vector<int> tb_chunks(unsigned n, int K)
+{
+    std::vector<int> out;
+    if (K <= 1) {
+        out.assign(n, 1);
+        return out;
+    }
+    unsigned q = n / (unsigned)K, r = n % (unsigned)K;
+    if (r == 1 && q > 0) {
+        q--;
+        r += (unsigned)K;
+    }
+    for (unsigned i = 0; i < q; i++) out.push_back(K);
+    if (r == 0) return out;
+    if (r == 1 || pd_tb_supported((int)r)) {
+        out.push_back((int)r);
+        return out;
+    }
+    for (unsigned a = (r + 1) / 2; a < r; a++)
+        if (pd_tb_supported((int)a) && pd_tb_supported((int)(r - a))) {
+            out.push_back((int)a);
+            out.push_back((int)(r - a));
+            return out;
+        }
+    for (unsigned i = 0; i < r; i++) out.push_back(1);
+    return out;
+}
 
 hipError_t launch_pyramid(const PyrArgs &a, hipStream_t s);
 hipError_t launch_assign_zeros(const LevelArgs &a, hipStream_t s);
