@@ -17,6 +17,7 @@ from .lib import (  # noqa: F401
     hip_library,
     lib_path,
 )
+from . import png  # noqa: F401
 from .synth import synth_pair, synth_pair_float, synth_pairs_float, synth_static_float, synth_sequence_float, synth_hash  # noqa: F401
 from .schedule import ctf_levels_for_rows, num_max_iter, gaussian_mask, level_table  # noqa: F401
 from .net import Net, mfma_probe  # noqa: F401

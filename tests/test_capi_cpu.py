@@ -23,6 +23,7 @@ def declared(header):
 @pytest.mark.parametrize("header,lib", [
     ("pdflow_hip.h", "libpdflow_hip.so"),
     ("pdflow_cnn.h", "libpdflow_hip.so"),
+    ("pdflow_png.h", "libpdflow_png.so"),
     ("pdflow_synth.h", "libpdflow_synth.so"),
     ("pdflow_cudalib.h", "libpdflow_csf.so"),
 ])

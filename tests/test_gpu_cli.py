@@ -1,0 +1,60 @@
+"""The command-line pair evaluator (pd-flow_amd/pdflow/pdflow-impair, the counterpart of the
+reference's Scene-Flow-Impair app) end to end: PNG inputs -> solve on the GPU -> the results
+text file and the RGB representation, against the single-pair solver."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from pdflow import Solver, png, synth_pair
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "pd-flow_amd", "pdflow", "pdflow-impair")
+
+
+def _write_inputs(d, seed, H, W):
+    i1, z1, i2, z2 = synth_pair(seed, H, W)
+    paths = {}
+    for name, plane in (("i1", i1), ("z1", z1), ("i2", i2), ("z2", z2)):
+        paths[name] = str(d / f"{name}.png")
+        png.write(paths[name], np.ascontiguousarray(plane.reshape(W, H).T))   # column-major -> image
+    return paths
+
+
+@pytest.mark.parametrize("rows,H,W", [(240, 240, 320), (120, 480, 640)])
+def test_cli_matches_the_solver(gpu_available, tmp_path, rows, H, W):
+    paths = _write_inputs(tmp_path, 77 + rows, H, W)
+    root = str(tmp_path / "out")
+    cmd = [CLI, "--rows", str(rows), "--i1", paths["i1"], "--i2", paths["i2"], "--z1", paths["z1"],
+           "--z2", paths["z2"], "--out", root, "--no-show"]
+    print("running", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    cols = rows * 4 // 3
+    # the same frames through the single-pair solver (I = u8, Z = u16 / 5000, as loaded)
+    I1, Z1 = png.load_frame(paths["i1"], paths["z1"], W, H)
+    I2, Z2 = png.load_frame(paths["i2"], paths["z2"], W, H)
+    s = Solver(rows, 480 // H)
+    dx, dy, dz = s.solve_pair(I1, Z1, I2, Z2)
+    lines = open(root + "_results01.txt").read().splitlines()
+    assert len(lines) == rows * cols
+    # "v u dx dy dz", v-major (scene_flow_impair.cpp:331-339), std::ostream default formatting
+    for k in range(0, rows * cols, 97):
+        v, u = k // cols, k % cols
+        i = v + u * rows
+        want = f"{v} {u} {float(dx[i]):g} {float(dy[i]):g} {float(dz[i]):g}"
+        assert lines[k] == want, (k, lines[k], want)
+    img = png.read(root + "_representation01.png")
+    assert img.shape == (rows, cols, 3)
+    for plane, ch in ((dz, 0), (dy, 1), (dx, 2)):            # R = z, G = y, B = x
+        a = np.abs(plane.reshape(cols, rows).T)
+        m = a.max()
+        want = (np.float32(255.0) * a / m).astype(np.uint8) if m > 0 else np.zeros_like(a, np.uint8)
+        assert np.array_equal(img[..., ch], want)
+    # a second run takes the next free number
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and os.path.exists(root + "_results02.txt")
+    assert open(root + "_results02.txt").read() == open(root + "_results01.txt").read()
