@@ -110,6 +110,117 @@ __global__ void __launch_bounds__(256) k_conv3x3(ConvArgs a)
     }
 }
 
+// LDS-staged variant: a workgroup (4 waves) computes 128 pixels x WN output channels; each wave
+// 32 pixels (two 16-row MFMA tiles) x WN channels (WN/16 tiles), so every A fragment feeds WN/16
+// products and every B fragment two.  Per K chunk of 32 the im2col rows and the weight rows are
+// staged through LDS once for the whole workgroup.  Each thread's staging pixels and channels
+// are fixed across chunks, so their coordinates are computed once and the (tap, channel) split
+// of k advances incrementally.
+constexpr int CS_P = 128, CS_K = 32;
+
+template <int WN>
+__global__ void __launch_bounds__(256) k_conv3x3_lds(ConvArgs a)
+{
+    constexpr int NT = WN / 16;
+    constexpr int AG = CS_P * CS_K / 4 / 256;      // A groups of 4 k per thread per chunk (4)
+    constexpr int BG = (WN * CS_K / 4 + 255) / 256;
+    __shared__ uint16_t sA[CS_P][CS_K + 4];
+    __shared__ uint16_t sB[WN][CS_K + 4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int P = a.nb * a.h * a.w;
+    const int p0 = blockIdx.x * CS_P, co0 = blockIdx.y * WN;
+    const int kreal = 9 * a.cin;
+    const int kg = 4 * (tid % (CS_K / 4));          // this thread's k offset within a chunk
+    // staging pixels: rows tid/8 + 32 r of the tile
+    int sx[AG], sy[AG], sn[AG];
+    bool sok[AG];
+#pragma unroll
+    for (int r = 0; r < AG; r++) {
+        const int p = p0 + tid / (CS_K / 4) + 32 * r;
+        sok[r] = p < P;
+        const int pp = sok[r] ? p : 0;
+        sx[r] = pp % a.w;
+        sy[r] = (pp / a.w) % a.h;
+        sn[r] = pp / (a.w * a.h);
+    }
+    int tap = kg / a.cin, ci = kg - tap * a.cin;   // (tap, channel) of k = kc + kg
+    f32x4 acc[2][NT];
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // global -> registers for the chunk at kc (the next chunk's loads are in flight while the
+    // current chunk's products run), registers -> LDS at the top of each chunk
+    bf16x4 ra[AG], rb[BG];
+    auto gload = [&](int kc) {
+        const int k = kc + kg;
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+        for (int r = 0; r < AG; r++) {
+            ra[r] = bf16x4{0, 0, 0, 0};
+            const int yy = sy[r] + dy, xx = sx[r] + dx;
+            if (sok[r] && k < kreal && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
+                ra[r] = *reinterpret_cast<const bf16x4 *>(a.x + ((size_t)(sn[r] * a.h + yy) * a.w + xx) * a.cin + ci);
+        }
+#pragma unroll
+        for (int r = 0; r < BG; r++) {
+            const int e = tid + 256 * r, co = co0 + e / (CS_K / 4);
+            rb[r] = bf16x4{0, 0, 0, 0};
+            if (e < WN * CS_K / 4 && co < a.coutp && k < a.kp)
+                rb[r] = *reinterpret_cast<const bf16x4 *>(a.wt + (size_t)co * a.kp + k);
+        }
+        ci += CS_K;                          // (tap, channel) of the next chunk
+        while (ci >= a.cin) {
+            ci -= a.cin;
+            tap++;
+        }
+    };
+    gload(0);
+    for (int kc = 0; kc < a.kp; kc += CS_K) {
+#pragma unroll
+        for (int r = 0; r < AG; r++) *reinterpret_cast<bf16x4 *>(&sA[tid / (CS_K / 4) + 32 * r][kg]) = ra[r];
+#pragma unroll
+        for (int r = 0; r < BG; r++) {
+            const int e = tid + 256 * r;
+            if (e < WN * CS_K / 4) *reinterpret_cast<bf16x4 *>(&sB[e / (CS_K / 4)][kg]) = rb[r];
+        }
+        __syncthreads();
+        if (kc + CS_K < a.kp) gload(kc + CS_K);
+#pragma unroll
+        for (int ks = 0; ks < CS_K; ks += 16) {
+            const int kq = ks + 4 * (lane / 16);
+            bf16x4 af[2], bfr[NT];
+#pragma unroll
+            for (int m = 0; m < 2; m++) af[m] = *reinterpret_cast<const bf16x4 *>(&sA[wave * 32 + m * 16 + lane % 16][kq]);
+#pragma unroll
+            for (int t = 0; t < NT; t++) bfr[t] = *reinterpret_cast<const bf16x4 *>(&sB[t * 16 + lane % 16][kq]);
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int t = 0; t < NT; t++) acc[m][t] = mfma(af[m], bfr[t], acc[m][t]);
+        }
+        __syncthreads();
+    }
+    // epilogue: lane holds channel co0 + 16 t + lane % 16 for pixels (tile m) 4 * (lane / 16) + j
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int co = co0 + 16 * t + lane % 16;
+        if (co >= a.cout) continue;
+        const float bias = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+        for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int p = p0 + wave * 32 + m * 16 + 4 * (lane / 16) + j;
+                if (p >= P) continue;
+                float v = acc[m][t][j] + bias;
+                if (a.relu) v = v > 0.f ? v : 0.f;
+                if (a.y16) a.y16[(size_t)p * a.ystride + a.yoff + co] = f2bf(v);
+                if (a.y32) a.y32[(size_t)p * a.ystride + a.yoff + co] = v;
+            }
+    }
+}
+
 // ------------------------------------------------------------------ weight gradient
 // dwt[co][k] += sum over pixels of dz[p][co] * xcol[p][k].  WG: 256 threads -> 32 channels x
 // 64 k over a chunk of pixels, staged 64 pixels at a time through LDS in transposed form so
@@ -373,6 +484,20 @@ hipError_t probe(const uint16_t *A, const uint16_t *B, float *C, hipStream_t s)
 }
 
 hipError_t conv3x3(const ConvArgs &a, hipStream_t s)
+{
+    const int P = a.nb * a.h * a.w;
+    if (a.cin % 4 || a.kp % 16) return hipErrorInvalidValue;
+    const unsigned gx = (unsigned)((P + CS_P - 1) / CS_P);
+    if (a.coutp >= 64)
+        hipLaunchKernelGGL(k_conv3x3_lds<64>, dim3(gx, (a.coutp + 63) / 64), dim3(256), 0, s, a);
+    else if (a.coutp >= 32)
+        hipLaunchKernelGGL(k_conv3x3_lds<32>, dim3(gx, (a.coutp + 31) / 32), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_conv3x3_lds<16>, dim3(gx, (a.coutp + 15) / 16), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t conv3x3_direct(const ConvArgs &a, hipStream_t s)
 {
     const int P = a.nb * a.h * a.w;
     hipLaunchKernelGGL(k_conv3x3, dim3((P + CT_P - 1) / CT_P, (a.coutp + CT_C - 1) / CT_C), dim3(256), 0, s, a);

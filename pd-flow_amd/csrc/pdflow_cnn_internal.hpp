@@ -27,7 +27,8 @@ struct WGradArgs {
 };
 
 hipError_t probe(const uint16_t *A, const uint16_t *B, float *C, hipStream_t s);
-hipError_t conv3x3(const ConvArgs &a, hipStream_t s);
+hipError_t conv3x3(const ConvArgs &a, hipStream_t s);          // LDS-staged (used)
+hipError_t conv3x3_direct(const ConvArgs &a, hipStream_t s);   // one 64x32 tile per workgroup, no LDS
 hipError_t conv3x3_wgrad(const WGradArgs &a, hipStream_t s);
 hipError_t bias_grad(const uint16_t *dz, float *db, int P, int coutp, int cout, hipStream_t s);
 hipError_t relu_grad(const uint16_t *dy, const uint16_t *y, uint16_t *dz, size_t n, hipStream_t s);
