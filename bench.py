@@ -27,11 +27,12 @@ import numpy as np  # noqa: E402
 METRIC = "VGA scene-flow pairs/s: batched+learned-init vs plain, same EPE; HBM % peak"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PIXEL_ITER = 148     # 11 constant planes read + 13 state planes read and written, FP32
-# HBM bytes per kb_pd_tb<4> launch over the learned solve's launch mix at 256 VGA pairs, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of this workload, scaled by the counters'
-# readings for a plain copy of known size (fetch x 2.0, write x 1.0); the passes and the per-level
-# figures are in profiles/pd_iteration_batched_learned_vga256_hbm_traffic.txt
-TRAFFIC_TB4_LEARNED_VGA256 = 8.250e9
+# HBM bytes per temporally blocked launch (kb_pd_tb<K, 32, 32>, K = 4 and 5) over the learned
+# solve's launch mix at 256 VGA pairs, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of this
+# workload, scaled by the counters' readings for a plain copy of known size (fetch x 2.0, write
+# x 1.0); the passes and the per-level figures are in
+# profiles/pd_iteration_batched_learned_vga256_hbm_traffic.txt
+TRAFFIC_TB_LEARNED_VGA256 = 4.885e9
 
 
 def log(msg):
@@ -427,7 +428,7 @@ def run_learned(args):
         return {"mean": round(float(np.mean(a)), 6), "median": round(float(np.median(a)), 6),
                 "p90": round(float(np.percentile(a, 90)), 6), "max": round(float(np.max(a)), 6)}
 
-    # roofline of the dominant kernel, the temporally blocked iteration kb_pd_tb<4>: HIP events
+    # roofline of the dominant kernel, the temporally blocked iteration kb_pd_tb<K>: HIP events
     # around each of its launches on the batch's stream, over eager passes of the learned solve
     learned.enable_kernel_timing(True, tb_only=True)
     for _ in range(3):
@@ -464,10 +465,10 @@ def run_learned(args):
                       "epe_vs_truth_ablation": stats(gt_abl),
                       "energy_ratio_learned": stats(le / pe), "residual_ratio_learned": stats(lr_ / pr),
                       "fallback_fraction": round(float(flags.mean()), 4)},
-        "roofline": {"bound": "hbm", "kernel": "kb_pd_tb<4> (batched, 4 primal-dual iterations per launch)",
+        "roofline": {"bound": "hbm", "kernel": "kb_pd_tb<K, 32, 32> (batched, K = 4 or 5 primal-dual iterations per launch)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": (TRAFFIC_TB4_LEARNED_VGA256
+                     "traffic": (TRAFFIC_TB_LEARNED_VGA256
                                  if (rows, n, args.factor) == (480, 256, 4) else None),
                      "traffic_algorithmic_bytes": round(BYTES_PER_PIXEL_ITER * pix / launches),
                      "launches": launches,
