@@ -327,7 +327,7 @@ void build_plan(pdf_batch *b, const std::vector<unsigned> &who, bool fallback, P
             for (unsigned i : ids) {
                 const Pair &p = b->pairs[i];
                 args.push_back(level_args(b, p));
-                blocks.push_back(pdf::batch_blocks_per_level(p.rows_i * p.cols_i));
+                blocks.push_back(pdf::batch_blocks(k, p.rows_i, p.cols_i));
             }
             add_launch(plan, setup_launch(k, flag), args, blocks);
         };

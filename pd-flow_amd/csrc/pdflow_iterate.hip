@@ -300,8 +300,7 @@ __device__ __forceinline__ void primal_px(Px &q, int rows, int cols, float riu, 
 //   shape 0: 64 x 32, 1024 threads (one workgroup per CU: 88 KB LDS, ~118 VGPRs)
 //   shape 1: 32 x 32,  512 threads (two workgroups per CU, so one's loads overlap the other's
 //            arithmetic; more halo per output pixel)
-constexpr int BV = 64, BU = 32;        // shape 0, also the whole-level kernel's thread layout
-constexpr int BT = 1024;
+constexpr int BT = 1024;               // threads of the whole-level kernel
 
 // applies f(pixel, j) to a thread's two pixels (named variables, so they stay in registers)
 #define PD_BOTH(f) do { f(x0, 0); f(x1, 1); } while (0)

@@ -33,55 +33,80 @@ __constant__ int kCarry[10] = {PDF_DU_L, PDF_DV_L, PDF_DW_L, PDF_PD_L, PDF_PUU_L
 __constant__ int kUpDst[10] = {PDF_DU_PREV, PDF_DV_PREV, PDF_DW_NEW, PDF_PD, PDF_PUU,
                                PDF_PUV, PDF_PVU, PDF_PVV, PDF_PWU, PDF_PWV};
 
-__device__ __forceinline__ void level_init_body(const LevelArgs &a, unsigned block, int has_coarse)
+// 2-D tiles of 32 rows x 8 columns (256 threads, lanes down a column) for the set-up kernels
+// that read a neighbourhood: the neighbourhood is staged through LDS once per tile.
+constexpr int kTR = 32, kTC = 8;
+__host__ __device__ inline unsigned tiles_of(unsigned rows, unsigned cols)
 {
-    const unsigned index = block * kBlock + threadIdx.x;
+    return ((rows + kTR - 1) / kTR) * ((cols + kTC - 1) / kTC);
+}
+
+__device__ __forceinline__ void level_init_body(const LevelArgs &a, unsigned tile, int has_coarse)
+{
+    // the coarse patch a tile's 5x5 even taps touch: coarse rows v0/2-1 .. v0/2+16, columns
+    // u0/2-1 .. u0/2+4 (clamped to the coarse level)
+    __shared__ float sC[10][19 * 7];
     const unsigned rows_i = a.rows, cols_i = a.cols;
-    if (index >= rows_i * cols_i) return;
-    // .cu:447-454: du_new, dv_new, du_acc, dv_acc start at 0 on every level
-#pragma unroll
-    for (int k = 0; k < 4; k++) a.f[kInitZero[k]][index] = 0.f;
+    const unsigned tv = (rows_i + kTR - 1) / kTR;
+    const int v0 = (int)(tile % tv) * kTR, u0 = (int)(tile / tv) * kTC;
+    const int v = v0 + (int)threadIdx.x % kTR, u = u0 + (int)threadIdx.x / kTR;
+    const bool on = v < (int)rows_i && u < (int)cols_i;
+    const unsigned index = on ? (unsigned)v + (unsigned)u * rows_i : 0u;
     float acc[10];
 #pragma unroll
     for (int q = 0; q < 10; q++) acc[q] = 0.f;
     if (has_coarse) {
-        // .cu:460-569: the zero-inserted plane holds the coarse value at even (v, u) (du, dv
-        // doubled) and +0 elsewhere.  A running sum that starts at +0 is never -0 under
-        // round-to-nearest, so adding the +0 products of odd taps leaves it unchanged: only
-        // the even taps are visited, in the reference's (k outer, l inner) order.
-        const int v = (int)(index % rows_i), u = (int)(index / rows_i), rc = (int)rows_i / 2;
-        const bool inner = (v > 1) && (v < (int)rows_i - 2) && (u > 1) && (u < (int)cols_i - 2);
-        for (int k = (v & 1) ? -1 : -2; k < 3; k += 2) {
-            const int vv = v + k;
-            if (vv < 0 || vv >= (int)rows_i) continue;
-            for (int l = (u & 1) ? -1 : -2; l < 3; l += 2) {
-                const int uu = u + l;
-                if (uu < 0 || uu >= (int)cols_i) continue;
-                const float g = 4.f * a.g[12 + k + 5 * l];
-                const unsigned ci = (unsigned)(vv / 2) + (unsigned)(uu / 2) * (unsigned)rc;
+        const int rc = (int)rows_i / 2, cc = (int)cols_i / 2;
+        const int cr0 = v0 >= 2 ? (v0 - 2) / 2 : 0, cr1 = min(rc - 1, (v0 + kTR + 1) / 2);
+        const int cc0 = u0 >= 2 ? (u0 - 2) / 2 : 0, cc1 = min(cc - 1, (u0 + kTC + 1) / 2);
+        const int CR = cr1 - cr0 + 1, CC = cc1 - cc0 + 1;
+        for (int e = (int)threadIdx.x; e < 10 * CR * CC; e += kBlock) {
+            const int q = e / (CR * CC), r = e - q * (CR * CC);
+            sC[q][r] = a.f[kCarry[q]][(unsigned)(cr0 + r % CR) + (unsigned)(cc0 + r / CR) * (unsigned)rc];
+        }
+        __syncthreads();
+        if (on) {
+            // .cu:460-569: the zero-inserted plane holds the coarse value at even (v, u) (du, dv
+            // doubled) and +0 elsewhere.  A running sum that starts at +0 is never -0 under
+            // round-to-nearest, so adding the +0 products of odd taps leaves it unchanged: only
+            // the even taps are visited, in the reference's (k outer, l inner) order.
+            const bool inner = (v > 1) && (v < (int)rows_i - 2) && (u > 1) && (u < (int)cols_i - 2);
+            for (int k = (v & 1) ? -1 : -2; k < 3; k += 2) {
+                const int vv = v + k;
+                if (vv < 0 || vv >= (int)rows_i) continue;
+                for (int l = (u & 1) ? -1 : -2; l < 3; l += 2) {
+                    const int uu = u + l;
+                    if (uu < 0 || uu >= (int)cols_i) continue;
+                    const float g = 4.f * a.g[12 + k + 5 * l];
+                    const int ci = (vv / 2 - cr0) + (uu / 2 - cc0) * CR;
 #pragma unroll
-                for (int q = 0; q < 10; q++) {
-                    const float cval = a.f[kCarry[q]][ci];
-                    acc[q] += g * (q < 2 ? 2.f * cval : cval);
+                    for (int q = 0; q < 10; q++) {
+                        const float cval = sC[q][ci];
+                        acc[q] += g * (q < 2 ? 2.f * cval : cval);
+                    }
                 }
             }
-        }
-        if (!inner) {
-            // border weight: 1 - sum of g over the out-of-range taps (.cu:515-542)
-            float aw = 1.f;
+            if (!inner) {
+                // border weight: 1 - sum of g over the out-of-range taps (.cu:515-542)
+                float aw = 1.f;
 #pragma unroll
-            for (int k = -2; k < 3; k++)
+                for (int k = -2; k < 3; k++)
 #pragma unroll
-                for (int l = -2; l < 3; l++) {
-                    const int vv = v + k, uu = u + l;
-                    if (vv < 0 || vv >= (int)rows_i || uu < 0 || uu >= (int)cols_i)
-                        aw -= 0.25f * (4.f * a.g[12 + k + 5 * l]);
-                }
-            const float inv = 1.f / aw;
+                    for (int l = -2; l < 3; l++) {
+                        const int vv = v + k, uu = u + l;
+                        if (vv < 0 || vv >= (int)rows_i || uu < 0 || uu >= (int)cols_i)
+                            aw -= 0.25f * (4.f * a.g[12 + k + 5 * l]);
+                    }
+                const float inv = 1.f / aw;
 #pragma unroll
-            for (int q = 0; q < 10; q++) acc[q] *= inv;
+                for (int q = 0; q < 10; q++) acc[q] *= inv;
+            }
         }
     }
+    if (!on) return;
+    // .cu:447-454: du_new, dv_new, du_acc, dv_acc start at 0 on every level
+#pragma unroll
+    for (int k = 0; k < 4; k++) a.f[kInitZero[k]][index] = 0.f;
 #pragma unroll
     for (int q = 0; q < 10; q++) a.f[kUpDst[q]][index] = acc[q];
     a.f[PDF_DW_ACC][index] = acc[2];
@@ -230,14 +255,34 @@ __device__ __forceinline__ Sat saturated(const LevelArgs &a, unsigned i)
     return {du + a.f[PDF_DU_PREV][i], dv + a.f[PDF_DV_PREV][i], dw};
 }
 
-__device__ __forceinline__ void filter_motion_body(const LevelArgs &a, unsigned block)
+__device__ __forceinline__ void filter_motion_body(const LevelArgs &a, unsigned tile)
 {
-    const unsigned index = block * kBlock + threadIdx.x;
+    // the tile plus a one-pixel halo: saturated solution, frame-1 depth and ddt of each pixel,
+    // computed once per pixel and read by the (up to) 9 pixels whose window contains it
+    constexpr int HR = kTR + 2, HC = kTC + 2;
+    __shared__ float sU[HR * HC], sV[HR * HC], sW[HR * HC], sZ[HR * HC], sT[HR * HC];
     const unsigned rows_i = a.rows, cols_i = a.cols;
-    if (index >= rows_i * cols_i) return;
-    const unsigned v = index % rows_i, u = index / rows_i;
-    const float *zo = a.Zold;
-    const float d0 = zo[index];
+    const unsigned tv = (rows_i + kTR - 1) / kTR;
+    const int v0 = (int)(tile % tv) * kTR, u0 = (int)(tile / tv) * kTC;
+    for (int e = (int)threadIdx.x; e < HR * HC; e += kBlock) {
+        const int vv = v0 - 1 + e % HR, uu = u0 - 1 + e / HR;
+        if (vv >= 0 && vv < (int)rows_i && uu >= 0 && uu < (int)cols_i) {
+            const unsigned il = (unsigned)vv + (unsigned)uu * rows_i;
+            const Sat f = saturated(a, il);
+            sU[e] = f.du;
+            sV[e] = f.dv;
+            sW[e] = f.dw;
+            sZ[e] = a.Zold[il];
+            sT[e] = a.f[PDF_DDT][il];
+        }
+    }
+    __syncthreads();
+    const int lv = (int)threadIdx.x % kTR, lu = (int)threadIdx.x / kTR;
+    const int v = v0 + lv, u = u0 + lu;
+    if (v >= (int)rows_i || u >= (int)cols_i) return;
+    const unsigned index = (unsigned)v + (unsigned)u * rows_i;
+    const int hc = (lv + 1) + (lu + 1) * HR;            // this pixel in the halo tile
+    const float d0 = sZ[hc];
     float ou, ov, ow;
     if (d0 > 0.f) {
         // .cu:1057-1132: taps in the reference's order (k = v offset outer, l = u offset inner)
@@ -252,21 +297,21 @@ __device__ __forceinline__ void filter_motion_body(const LevelArgs &a, unsigned 
         for (int k = -1; k < 2; k++)
 #pragma unroll
             for (int l = -1; l < 2; l++) {
-                const int r = (int)v + k, c = (int)u + l;
+                const int r = v + k, c = u + l;
                 const bool ok = !(r < 0 || r >= (int)rows_i || c < 0 || c >= (int)cols_i);
-                const unsigned il = ok ? index + l * (int)rows_i + k : index;
-                const float dz = d0 - zo[il], dt = a.f[PDF_DDT][il];
+                const int h = ok ? hc + k + l * HR : hc;
+                const float dz = d0 - sZ[h], dt = sT[h];
                 const float pres = 1.f / (1.f + 5.f * (dz * dz) + 10.f * (dt * dt));
-                const Sat f = saturated(a, il);
+                const float fu = sU[h], fv = sV[h], fw = sW[h];
 #pragma unroll
                 for (int s = 0; s < 9; s++)
                     if (ok && s == n) {
-                        ku[s] = f.du; kv[s] = f.dv; kw[s] = f.dw;
+                        ku[s] = fu; kv[s] = fv; kw[s] = fw;
                         wu[s] = pres; wv[s] = pres; ww[s] = pres;
                     }
                 n += ok ? 1 : 0;
             }
-        sort9(ku, wu);
+    sort9(ku, wu);
         sort9(kv, wv);
         sort9(kw, ww);
         float cu[9], cv[9], cw[9];
@@ -282,8 +327,7 @@ __device__ __forceinline__ void filter_motion_body(const LevelArgs &a, unsigned 
         ov = wmedian_pick(kv, cv, med);
         ow = wmedian_pick(kw, cw, med);
     } else {
-        const Sat f = saturated(a, index);
-        ou = f.du; ov = f.dv; ow = f.dw;
+        ou = sU[hc]; ov = sV[hc]; ow = sW[hc];
     }
     a.f[PDF_DU_L][index] = ou;
     a.f[PDF_DV_L][index] = ov;
@@ -398,9 +442,8 @@ __global__ void __launch_bounds__(kBlock) kb_pyramid(BatchTable t)
 
 hipError_t launch_level_init(const LevelArgs &a, bool has_coarse, hipStream_t s)
 {
-    const unsigned n = a.rows * a.cols;
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_level_init, grid_for(n), dim3(kBlock), 0, s, a, has_coarse ? 1 : 0);
+    if (a.rows == 0 || a.cols == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_level_init, dim3(tiles_of(a.rows, a.cols)), dim3(kBlock), 0, s, a, has_coarse ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -430,9 +473,8 @@ hipError_t launch_warp_mu(const LevelArgs &a, bool write_steps, hipStream_t s)
 
 hipError_t launch_filter_motion(const LevelArgs &a, hipStream_t s)
 {
-    const unsigned n = a.rows * a.cols;
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_filter_motion, grid_for(n), dim3(kBlock), 0, s, a);
+    if (a.rows == 0 || a.cols == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_filter_motion, dim3(tiles_of(a.rows, a.cols)), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 
@@ -466,5 +508,11 @@ hipError_t launch_batch(BatchKernel k, const BatchTable &t, unsigned total_block
 }
 
 unsigned batch_blocks_per_level(unsigned pixels) { return (pixels + kBlock - 1) / kBlock; }
+
+unsigned batch_blocks(BatchKernel k, unsigned rows, unsigned cols)
+{
+    if (k == BatchKernel::LevelInit || k == BatchKernel::FilterMotion) return tiles_of(rows, cols);
+    return batch_blocks_per_level(rows * cols);
+}
 
 }  // namespace pdf
