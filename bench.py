@@ -81,6 +81,9 @@ def main():
     ap.add_argument("--batch-pairs", type=int, default=256)
     ap.add_argument("--factor", type=int, default=4, help="learned: iteration cut per level")
     ap.add_argument("--weights", default=os.path.join(ROOT, "pd-flow_amd", "pdflow", "weights", "warmstart_qvga.bin"))
+    ap.add_argument("--autotune", default=None, metavar="CACHE",
+                    help="batched/learned: pick the kernel configuration with pdf_batch_autotune "
+                         "(cached in the file CACHE) before timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run a plain device copy of BYTES three times (the calibration point "
                          "of HBM byte counters under rocprofv3 --pmc)")
@@ -273,6 +276,8 @@ def run_batched(args):
             distinct[key] = synth_pair_float(args.seed + 1000 * r + i % 16, 480 // cam, 640 // cam)
         b.set_inputs(i, *distinct[key])
     log(f"bench: {name}")
+    if args.autotune:
+        log(f"bench: autotuned (shape, K, ms) = {b.autotune(args.autotune)}")
     for _ in range(max(1, args.warmup)):
         b.solve(graph=not args.no_graph)
     b.synchronize()
@@ -369,6 +374,8 @@ def run_learned(args):
     pe, pr = plain.metrics()
     learned = make(cut)
     learned.set_learned(net)
+    if args.autotune:
+        log(f"bench: autotuned (shape, K, ms) = {learned.autotune(args.autotune)}")
     ablation = make(cut)
     inf = np.full(n, np.inf, np.float32)
     e_ref = {"energy_residual": pe, "residual": inf, "none": inf}[args.fallback]

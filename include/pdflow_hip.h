@@ -201,6 +201,13 @@ const char *pdf_batch_last_error(const pdf_batch *b);   /* NULL b: last create f
 int pdf_batch_set_iteration_kernel(pdf_batch *b, int k, unsigned min_pixels, int level_kernel);
 /* region shape of the temporally blocked kernel for the batch (as pdf_set_tb_shape) */
 int pdf_batch_set_tb_shape(pdf_batch *b, int shape);
+/* Autotuner: times the batch's own solve (one captured solve, then two graph replays between HIP
+ * events) for each region shape and iterations per launch K in {3, 4, 5, 6}, keeps the fastest and
+ * returns it (any output may be NULL; *ms = one solve).  Results are identical for every choice.
+ * With cache_path, a text file of "<key>\t<shape> <K> <ms>" lines keyed by device and batch
+ * composition: a hit applies the cached choice without timing, a miss appends the new line.  The
+ * batch's inputs must be set (it is solved several times). */
+int pdf_batch_autotune(pdf_batch *b, const char *cache_path, int *shape, int *k, float *ms);
 /* device pointers of pair's four input planes (the caller may fill them directly) */
 int pdf_batch_inputs_device(pdf_batch *b, unsigned pair, float **I1, float **Z1, float **I2,
                             float **Z2);

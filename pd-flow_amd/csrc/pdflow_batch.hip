@@ -23,7 +23,10 @@
 
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -516,6 +519,31 @@ int level_of_size(const Pair &p, unsigned h, unsigned w)
 
 }  // namespace
 
+namespace {
+
+// the autotuner's cache key: device and batch composition (sizes, schedules, warm/learned starts)
+std::string tune_key(const pdf_batch *b)
+{
+    int dev = 0;
+    hipDeviceProp_t prop;
+    std::string key = "dev=";
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        key += std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
+    std::map<std::string, unsigned> comp;
+    for (const Pair &p : b->pairs) {
+        std::string k = std::to_string(p.rows) + "x" + std::to_string(p.cam_mode) + ":s" + std::to_string(p.start_level) +
+                        (p.learned ? "L" : "") + ":";
+        for (unsigned l = 0; l < p.ctf_levels; l++) k += std::to_string(p.iters[l]) + (l + 1 < p.ctf_levels ? "," : "");
+        comp[k]++;
+    }
+    key += " pairs=";
+    for (const auto &kv : comp) key += kv.first + "*" + std::to_string(kv.second) + ";";
+    key += " min=" + std::to_string(b->tb_min_pixels) + " level=" + std::to_string(b->level_kernel ? 1 : 0);
+    return key;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *pdf_batch_last_error(const pdf_batch *b) { return b ? b->err.c_str() : g_batch_create_error.c_str(); }
@@ -653,6 +681,81 @@ int pdf_batch_set_tb_shape(pdf_batch *b, int shape)
         drop_plan(b);
         b->tb_shape = shape;
     }
+    return PDF_OK;
+}
+
+int pdf_batch_autotune(pdf_batch *b, const char *cache_path, int *shape_out, int *k_out, float *ms_out)
+{
+    if (!b) return PDF_ERR_ARG;
+    const std::string key = tune_key(b);
+    std::vector<std::string> lines;
+    if (cache_path) {
+        std::ifstream in(cache_path);
+        std::string line;
+        while (std::getline(in, line)) {
+            const size_t tab = line.find('\t');
+            if (tab != std::string::npos && line.compare(0, tab, key) == 0) {
+                int shape = -1, k = -1;
+                float ms = 0.f;
+                if (std::sscanf(line.c_str() + tab + 1, "%d %d %f", &shape, &k, &ms) == 3 &&
+                    pdf::pd_tb_shape_supported(shape) && pdf::pd_tb_supported(k)) {
+                    int rc = pdf_batch_set_tb_shape(b, shape);
+                    if (!rc) rc = pdf_batch_set_iteration_kernel(b, k, b->tb_min_pixels, b->level_kernel ? 1 : 0);
+                    if (rc) return rc;
+                    if (shape_out) *shape_out = shape;
+                    if (k_out) *k_out = k;
+                    if (ms_out) *ms_out = ms;
+                    return PDF_OK;
+                }
+            }
+            lines.push_back(line);
+        }
+    }
+    // time every candidate: one captured solve to build the plan and graph, then two replays
+    // between HIP events on the batch's stream
+    static const int kShapes[] = {pdf::kTBShape32x32, pdf::kTBShape64x32};
+    static const int kKs[] = {3, 4, 5, 6};
+    int best_shape = b->tb_shape, best_k = b->tb_k;
+    float best = 1e30f;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BHIP(b, hipEventCreate(&e0));
+    BHIP(b, hipEventCreate(&e1));
+    int rc = PDF_OK;
+    for (int shape : kShapes) {
+        for (int k : kKs) {
+            rc = pdf_batch_set_tb_shape(b, shape);
+            if (!rc) rc = pdf_batch_set_iteration_kernel(b, k, b->tb_min_pixels, b->level_kernel ? 1 : 0);
+            if (!rc) rc = pdf_batch_solve(b, 1);
+            if (rc) break;
+            if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = bfail(b, PDF_ERR_HIP, "event record"); break; }
+            for (int r = 0; r < 2 && !rc; r++) rc = pdf_batch_solve(b, 1);
+            if (rc) break;
+            float ms = 0.f;
+            if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = bfail(b, PDF_ERR_HIP, "event timing");
+                break;
+            }
+            ms *= 0.5f;
+            if (ms < best) { best = ms; best_shape = shape; best_k = k; }
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    rc = pdf_batch_set_tb_shape(b, best_shape);
+    if (!rc) rc = pdf_batch_set_iteration_kernel(b, best_k, b->tb_min_pixels, b->level_kernel ? 1 : 0);
+    if (rc) return rc;
+    if (cache_path) {
+        std::ofstream out(cache_path, std::ios::trunc);
+        for (const std::string &l : lines) out << l << "\n";
+        out << key << "\t" << best_shape << " " << best_k << " " << best << "\n";
+        if (!out) return bfail(b, PDF_ERR_ARG, std::string("cannot write ") + cache_path);
+    }
+    if (shape_out) *shape_out = best_shape;
+    if (k_out) *k_out = best_k;
+    if (ms_out) *ms_out = best;
     return PDF_OK;
 }
 

@@ -122,6 +122,7 @@ def hip_library():
         "pdf_device_copy": ([vp, vp, ctypes.c_size_t], c_int),
         "pdf_set_tb_shape": ([vp, c_int], c_int),
         "pdf_batch_set_tb_shape": ([vp, c_int], c_int),
+        "pdf_batch_autotune": ([vp, ctypes.c_char_p, ctypes.POINTER(c_int), ctypes.POINTER(c_int), fp], c_int),
         "pdf_batch_field_device": ([vp, c_uint, c_int, ctypes.POINTER(vp)], c_int),
         "pdf_batch_pyramid_device": ([vp, c_uint, c_int, c_uint, ctypes.POINTER(vp), up_, up_], c_int),
         "pdf_batch_set_learned": ([vp, vp, up_, c_uint], c_int),
@@ -398,6 +399,14 @@ class Batch:
 
     def set_tb_shape(self, shape):
         self._chk(self.lib.pdf_batch_set_tb_shape(self.ctx, shape))
+
+    def autotune(self, cache_path=None):
+        """picks the fastest region shape and iterations per launch for this batch (see
+        pdf_batch_autotune); returns (shape, k, ms per solve)"""
+        shape, k, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        self._chk(self.lib.pdf_batch_autotune(self.ctx, str(cache_path).encode() if cache_path else None,
+                                              ctypes.byref(shape), ctypes.byref(k), ctypes.byref(ms)))
+        return shape.value, k.value, ms.value
 
     def set_inputs(self, pair, I1, Z1, I2, Z2):
         a = [np.ascontiguousarray(x, np.float32) for x in (I1, Z1, I2, Z2)]

@@ -58,3 +58,32 @@ def test_batch_with_custom_schedules(gpu_available):
         b.synchronize()
         for a, w in zip(b.flow(i), flow):
             np.testing.assert_array_equal(a, w)
+
+
+def test_autotune_picks_a_config_and_caches_it(gpu_available, tmp_path):
+    """The autotuner times every region shape and K on the batch itself, keeps the fastest,
+    and a second batch of the same composition takes the cached choice; the flow is the same
+    for every choice."""
+    rows = [480, 240, 120]
+    say("autotune on a 3-pair batch")
+    cache = tmp_path / "tune.txt"
+    b = Batch(rows)
+    want = []
+    for i, r in enumerate(rows):
+        inputs, flow = single(r, 900 + i)
+        b.set_inputs(i, *inputs)
+        want.append(flow)
+    shape, k, ms = b.autotune(cache)
+    assert shape in (0, 1) and k in (3, 4, 5, 6) and ms > 0
+    b.solve(graph=True)
+    b.synchronize()
+    for i in range(len(rows)):
+        for a, w in zip(b.flow(i), want[i]):
+            np.testing.assert_array_equal(a, w)
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1 and lines[0].split("\t")[1].split()[:2] == [str(shape), str(k)]
+    b2 = Batch(rows)
+    for i, r in enumerate(rows):
+        b2.set_inputs(i, *single(r, 900 + i)[0])
+    assert b2.autotune(cache)[:2] == (shape, k)
+    assert len(cache.read_text().splitlines()) == 1
