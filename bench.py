@@ -2,7 +2,7 @@
 
 Default workload (--workload learned): 256 VGA (480x640) RGB-D pairs per step, batched with the
 learned warm start -- both pyramids of every pair, one network pass over their 60x80 levels,
-the solves from there with every level's iterations divided by 4, then the fallback rule (final
+the solves from there with every level's iterations divided by --factor (2), then the fallback rule (final
 energy or warped residual more than 5% above the plain solver's) and the plain re-solve of the
 pairs it rejects, all inside the timed region.  The line also carries the plain single-pair and
 plain batched rates of the same pairs, end-point errors against the plain solve and against the
@@ -79,7 +79,9 @@ def main():
                          "learned: --batch-pairs pairs of --rows per step, network warm start at "
                          "60x80, iterations / --factor, fallback rule")
     ap.add_argument("--batch-pairs", type=int, default=256)
-    ap.add_argument("--factor", type=int, default=4, help="learned: iteration cut per level")
+    ap.add_argument("--factor", type=float, default=2,
+                    help="learned: every level's iteration count divided by this (2 gives the highest "
+                         "throughput once the fallback re-solves are counted)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "pd-flow_amd", "pdflow", "weights", "warmstart_qvga.bin"))
     ap.add_argument("--autotune", default=None, metavar="CACHE",
                     help="batched/learned: pick the kernel configuration with pdf_batch_autotune "

@@ -117,7 +117,8 @@ def train(net, data, steps=20000, batch=64, lr=1e-3, lr_min=1e-5, seed=0, log=pr
 
 
 def cut_schedule(rows, factor):
-    return [max(1, n // factor) for n in schedule.num_max_iter(schedule.ctf_levels_for_rows(rows))]
+    """the reference schedule with every level's count divided by factor (rounded down, >= 1)"""
+    return [max(1, int(n / factor)) for n in schedule.num_max_iter(schedule.ctf_levels_for_rows(rows))]
 
 
 def _gt_epe(batch, i, gt, Z1):
