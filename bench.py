@@ -27,12 +27,12 @@ import numpy as np  # noqa: E402
 METRIC = "VGA scene-flow pairs/s: batched+learned-init vs plain, same EPE; HBM % peak"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PIXEL_ITER = 148     # 11 constant planes read + 13 state planes read and written, FP32
-# HBM bytes per temporally blocked launch (kb_pd_tb<K, 32, 32>, K = 4 and 5) over the learned
-# solve's launch mix at 256 VGA pairs, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of this
-# workload, scaled by the counters' readings for a plain copy of known size (fetch x 2.0, write
-# x 1.0); the passes and the per-level figures are in
+# HBM bytes per temporally blocked launch (kb_pd_tb<K, 32, 32>) over the learned solve's launch mix
+# at 256 VGA pairs, by iteration cut factor, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes
+# of this workload, scaled by the counters' readings for a plain copy of known size (fetch x 2.0,
+# write x 1.0); the passes and the per-level figures are in
 # profiles/pd_iteration_batched_learned_vga256_hbm_traffic.txt
-TRAFFIC_TB_LEARNED_VGA256 = 4.885e9
+TRAFFIC_TB_LEARNED_VGA256 = {2.0: 4.786e9, 4.0: 4.885e9}
 
 
 def log(msg):
@@ -474,11 +474,11 @@ def run_learned(args):
                       "epe_vs_truth_ablation": stats(gt_abl),
                       "energy_ratio_learned": stats(le / pe), "residual_ratio_learned": stats(lr_ / pr),
                       "fallback_fraction": round(float(flags.mean()), 4)},
-        "roofline": {"bound": "hbm", "kernel": "kb_pd_tb<K, 32, 32> (batched, K = 4 or 5 primal-dual iterations per launch)",
+        "roofline": {"bound": "hbm", "kernel": "kb_pd_tb<K, 32, 32> (batched, K = 2 to 5 primal-dual iterations per launch)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": (TRAFFIC_TB_LEARNED_VGA256
-                                 if (rows, n, args.factor) == (480, 256, 4) else None),
+                     "traffic": (TRAFFIC_TB_LEARNED_VGA256.get(float(args.factor))
+                                 if (rows, n) == (480, 256) else None),
                      "traffic_algorithmic_bytes": round(BYTES_PER_PIXEL_ITER * pix / launches),
                      "launches": launches,
                      "avg_launch_us": round(kms / launches * 1e3, 3),

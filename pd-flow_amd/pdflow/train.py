@@ -193,7 +193,7 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--eval-pairs", type=int, default=2000)
-    ap.add_argument("--factor", type=int, default=4)
+    ap.add_argument("--factor", type=float, default=4)
     ap.add_argument("--threads", type=int, default=None)
     ap.add_argument("--init", default=None, help="weights to start from (resume)")
     ap.add_argument("--out", default="warmstart.bin")
