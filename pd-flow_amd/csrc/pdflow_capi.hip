@@ -828,4 +828,24 @@ int pdf_device_copy(void *dst, const void *src, size_t bytes)
     return hipDeviceSynchronize() == hipSuccess ? PDF_OK : PDF_ERR_HIP;
 }
 
+int pdf_check_exact_sequence(int which, uint32_t lo, uint32_t hi, uint64_t *mismatches, uint32_t *first_mismatch)
+{
+    if (which < 0 || which > 2 || lo > hi || !mismatches || !first_mismatch) return PDF_ERR_ARG;
+    void *d = nullptr;
+    if (hipMalloc(&d, 16) != hipSuccess) return PDF_ERR_HIP;
+    const uint32_t init[4] = {0u, 0u, 0xffffffffu, 0u};
+    int st = PDF_OK;
+    uint32_t out[4];
+    if (hipMemcpy(d, init, 16, hipMemcpyHostToDevice) != hipSuccess ||
+        pdf::check_exact(which, lo, hi, (unsigned long long *)d, (unsigned *)((char *)d + 8)) != hipSuccess ||
+        hipMemcpy(out, d, 16, hipMemcpyDeviceToHost) != hipSuccess)
+        st = PDF_ERR_HIP;
+    hipFree(d);
+    if (st == PDF_OK) {
+        *mismatches = (uint64_t)out[0] | ((uint64_t)out[1] << 32);
+        *first_mismatch = out[2];
+    }
+    return st;
+}
+
 }  // extern "C"

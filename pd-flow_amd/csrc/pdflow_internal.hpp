@@ -171,5 +171,6 @@ unsigned pd_blocks(int k, int shape, unsigned rows, unsigned cols);
 hipError_t launch_motion_field(const LevelArgs &a, hipStream_t s);
 // device-to-device streaming copy (bytes a multiple of 16, 16-byte aligned pointers)
 hipError_t launch_stream_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
+hipError_t check_exact(int which, unsigned lo, unsigned hi, unsigned long long *count, unsigned *first);
 
 }  // namespace pdf

@@ -195,13 +195,13 @@ struct Px {
 
 __device__ __forceinline__ void load_px(Px &q, const PDArgs &a, int gv, int gu, bool on)
 {
-    // loads are unconditional (pixel 0 stands in for pixels outside the image) and then
-    // selected, so every value stays in a register
+    // loads are unconditional: pixels outside the image load pixel 0's values, which no pixel
+    // inside the image reads (every image-border term is selected away by gv/gu)
     q.gv = gv;
     q.gu = gu;
     q.on = on;
     const unsigned g = on ? (unsigned)gv + (unsigned)gu * a.rows : 0u;
-    auto ld = [&](const float *plane, float off) { return on ? plane[g] : off; };
+    auto ld = [&](const float *plane, float) { return plane[g]; };
     q.du = ld(a.in[S_DU], 0.f); q.dv = ld(a.in[S_DV], 0.f); q.dw = ld(a.in[S_DW], 0.f);
     q.ua = ld(a.in[S_DU_ACC], 0.f); q.va = ld(a.in[S_DV_ACC], 0.f); q.wa = ld(a.in[S_DW_ACC], 0.f);
     q.pd = ld(a.in[S_PD], 0.f);
@@ -300,7 +300,38 @@ __device__ __forceinline__ void primal_px(Px &q, int rows, int cols, float riu, 
 //   shape 0: 64 x 32, 1024 threads (one workgroup per CU: 88 KB LDS, ~118 VGPRs)
 //   shape 1: 32 x 32,  512 threads (two workgroups per CU, so one's loads overlap the other's
 //            arithmetic; more halo per output pixel)
+//
+// Work that no output needs is skipped a whole wave at a time.  With d a pixel's distance to
+// the region's nearest edge (in rows or columns), iteration `it` (0-based) needs the dual step
+// at d >= it and the primal step at d >= it + 1: the dual at (v, u) reads ubar at (v, u+1) and
+// (v+1, u), the primal reads the new duals at (v, u-1) and (v-1, u), so each iteration widens
+// the band of wrong values by one pixel on every side, and the stored tile is d >= K.  The
+// column pairs are dealt to waves from both ends inwards (wave w holds pairs w and
+// RU/2 - 1 - w when a wave holds two), so the waves at the left and right edges hold only
+// small d and drop out after their first iterations.
 constexpr int BT = 1024;               // threads of the whole-level kernel
+
+// column pair (columns 2q, 2q+1 of the region) of slot s of wave w; S pairs per wave
+template <int S, int NP>
+__device__ __forceinline__ int tb_pair(int w, int s)
+{
+    static_assert(S == 1 || S == 2, "one or two column pairs per wave");
+    return (S == 1 || s == 0) ? w : NP - 1 - w;
+}
+
+// largest distance to the region edge over the pixels of wave w (it spans all RV rows)
+template <int RV, int RU>
+__device__ __forceinline__ int tb_wave_reach(int w)
+{
+    constexpr int S = 64 / RV, NP = RU / 2;
+    int reach = 0;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int c = 2 * tb_pair<S, NP>(w, s);
+        reach = max(reach, max(min(c, RU - 1 - c), min(c + 1, RU - 2 - c)));
+    }
+    return min(reach, (RV - 1) / 2);
+}
 
 // applies f(pixel, j) to a thread's two pixels (named variables, so they stay in registers)
 #define PD_BOTH(f) do { f(x0, 0); f(x1, 1); } while (0)
@@ -314,12 +345,16 @@ __device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned b
     __shared__ float sP[6][RU * RV];
     __shared__ float sRi[RU * RV], sRj[RU * RV];
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int S = 64 / RV, NP = RU / 2;
+    static_assert(RV * RU / 128 * S == NP, "waves x pairs per wave = column pairs");
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int rv0 = (int)bx * TVk - K, ru0 = (int)by * TUk - K;  // region origin
     const int rows = (int)a.rows, cols = (int)a.cols;
     const float li = a.lambda_i, ld = a.lambda_d;
     const int lv = lane % RV;
-    const int slot = wave * (64 / RV) + lane / RV;               // this thread's column pair
+    const int slot = tb_pair<S, NP>(wave, lane / RV);            // this thread's column pair
+    const int reach = tb_wave_reach<RV, RU>(wave);               // wave-uniform
 
     Px x0, x1;
     auto load = [&](Px &q, int j) __attribute__((always_inline)) {
@@ -345,9 +380,10 @@ __device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned b
         sV[idx] = q.va + q.vp;
         sW[idx] = q.wa;
     };
-    // pixels outside the image skip the arithmetic: no pixel inside reads them
+    // pixels outside the image run the same arithmetic on the values loaded for them; no pixel
+    // inside reads their results (the image-border terms are selected away), and without a
+    // branch the two pixels' instruction streams interleave
     auto dual = [&](Px &q, int j) __attribute__((always_inline)) {
-        if (!q.on) return;
         const int lu = 2 * slot + j, idx = lu * RV + lv;
         const int iu = lu + 1 < RU ? idx + RV : idx;
         const int iv = lv + 1 < RV ? idx + 1 : idx;
@@ -356,7 +392,6 @@ __device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned b
         for (int k = 0; k < 6; k++) sP[k][idx] = q.p[k];
     };
     auto primal = [&](Px &q, int j) __attribute__((always_inline)) {
-        if (!q.on) return;
         const int lu = 2 * slot + j, idx = lu * RV + lv;
         const int ju = lu > 0 ? idx - RV : idx;
         const int jv = lv > 0 ? idx - 1 : idx;
@@ -364,11 +399,11 @@ __device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned b
                   sP[3][jv], sP[5][jv], li, ld);
     };
     for (int it = 0; it < K; it++) {
-        PD_BOTH(put_bar);
+        if (reach >= it) PD_BOTH(put_bar);
         __syncthreads();
-        PD_BOTH(dual);
+        if (reach >= it) PD_BOTH(dual);
         __syncthreads();
-        PD_BOTH(primal);
+        if (reach > it) PD_BOTH(primal);
     }
 
     // write back the inner tile

@@ -533,4 +533,46 @@ hipError_t launch_stream_copy(const void *src, void *dst, size_t bytes, hipStrea
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ exact short sequences
+// Every float whose bit pattern lies in [lo, hi] goes through one of the short sequences of
+// pdflow_pixel.hpp and through the compiler's IEEE operation; out[0] counts the inputs whose
+// results differ in any bit, out[1] keeps the smallest such bit pattern (0xffffffff if none).
+__global__ void __launch_bounds__(256) k_check_exact(int which, unsigned lo, unsigned hi,
+                                                     unsigned long long *count, unsigned *first)
+{
+    unsigned long long bad = 0;
+    unsigned first_bad = 0xffffffffu;
+    const unsigned long long n = (unsigned long long)hi - lo + 1;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * 256) {
+        const unsigned bits = lo + (unsigned)i;
+        const float x = __uint_as_float(bits);
+        float fast, ieee;
+        if (which == 0) {
+            fast = sqrt_rn_gt1(x);
+            ieee = sqrtf(x);
+        } else if (which == 1) {
+            fast = rcp_rn_ge1(x);
+            ieee = 1.f / x;
+        } else {
+            fast = inv_sqrt_rn_gt1(x);
+            ieee = 1.f / sqrtf(x);
+        }
+        if (__float_as_uint(fast) != __float_as_uint(ieee)) {
+            bad++;
+            first_bad = min(first_bad, bits);
+        }
+    }
+    if (bad) {
+        atomicAdd(count, bad);
+        atomicMin(first, first_bad);
+    }
+}
+
+hipError_t check_exact(int which, unsigned lo, unsigned hi, unsigned long long *count, unsigned *first)
+{
+    hipLaunchKernelGGL(k_check_exact, dim3(256 * 32), dim3(256), 0, nullptr, which, lo, hi, count, first);
+    return hipGetLastError();
+}
+
 }  // namespace pdf

@@ -26,17 +26,55 @@ __device__ __forceinline__ float tau_w_of(float mu_uv, float lam, float acu_r)
     return 1.f / (mu_uv + lam * acu_r + 1e-10f);                         // .cu:807
 }
 
+// ------------------------------------------------------------------ exact short sequences
+// IEEE FP32 sqrt and division on gfx950 are v_sqrt_f32 / v_rcp_f32 estimates made exact by FMA
+// steps, wrapped in range handling: a 2^32 pre-scale and class test for sqrt, v_div_scale /
+// v_div_fmas / v_div_fixup for division.  On the ranges below the range handling is the
+// identity (no operand is tiny, huge, zero, denormal or non-finite), so these functions are the
+// same FMA steps without it and return the same bits as sqrtf and "/".
+// tests/test_gpu_exact.py sweeps every float of each range against sqrtf and "/".
+
+// sqrtf(x) for 1 < x <= FLT_MAX (and +inf): the estimate s, then the neighbour of s whose
+// residual x - s'*s shows that s was one step off (one below, one above).
+__device__ __forceinline__ float sqrt_rn_gt1(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float below = __int_as_float(__float_as_int(s) - 1);
+    const float above = __int_as_float(__float_as_int(s) + 1);
+    const float r_below = __builtin_fmaf(-below, s, x);
+    const float r_above = __builtin_fmaf(-above, s, x);
+    const float t = r_below <= 0.f ? below : s;
+    return r_above > 0.f ? above : t;
+}
+
+// 1.f / d for 1 <= d < 2^64: the v_rcp_f32 estimate refined by one Newton step, then two
+// residual corrections of the quotient.
+__device__ __forceinline__ float rcp_rn_ge1(float d)
+{
+    const float r0 = __builtin_amdgcn_rcpf(d);
+    const float r = __builtin_fmaf(__builtin_fmaf(-d, r0, 1.f), r0, r0);
+    const float q = __builtin_fmaf(__builtin_fmaf(-d, r, 1.f), r, r);
+    return __builtin_fmaf(__builtin_fmaf(-d, q, 1.f), r, q);
+}
+
+// 1.f / sqrtf(x) for x > 1 (1/+inf = 0)
+__device__ __forceinline__ float inv_sqrt_rn_gt1(float x)
+{
+    return x < __builtin_inff() ? rcp_rn_ge1(sqrt_rn_gt1(x)) : 0.f;
+}
+
 __device__ __forceinline__ void project(float &a, float &b)
 {
     // unit-disc projection (.cu:849-859): m = 1/sqrt(a*a+b*b), scale when m < 1.  With
     // correctly rounded sqrt and division, x = a*a+b*b <= 1 gives sqrt(x) <= 1 and so m >= 1:
-    // the square root and division are only evaluated when x > 1, with the same result for
-    // every input (NaN compares false either way).
+    // only x > 1 can scale, with the same result for every input (NaN compares false either
+    // way).  m is evaluated for every x and selected, so no lane branches; for x <= 1 it is
+    // outside inv_sqrt_rn_gt1's range and unused.
     const float x = a * a + b * b;
-    if (x > 1.f) {
-        const float m = 1.f / sqrtf(x);
-        if (m < 1.f) { a = a * m; b = b * m; }
-    }
+    const float m = inv_sqrt_rn_gt1(x);
+    const bool scale = x > 1.f && m < 1.f;
+    a = scale ? a * m : a;
+    b = scale ? b * m : b;
 }
 
 __device__ __forceinline__ float clamp_pd(float pd)
@@ -48,10 +86,10 @@ __device__ __forceinline__ float clamp_pd(float pd)
 
 __device__ __forceinline__ float div_axis(float r0, float p0, float rm, float pm, bool first, bool last)
 {
-    // one axis of .cu:941-979
-    if (first) return r0 * p0;
-    if (last) return -rm * pm;
-    return r0 * p0 - rm * pm;
+    // one axis of .cu:941-979, as selects (-rm * pm is the exact negation of rm * pm)
+    const float f = r0 * p0, g = rm * pm;
+    const float mid = f - g;
+    return first ? f : (last ? -g : mid);
 }
 
 struct Primal {

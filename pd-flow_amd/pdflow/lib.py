@@ -120,6 +120,8 @@ def hip_library():
         "pdf_stream_frame_device": ([vp, vp, vp, up, c_int, ctypes.POINTER(c_int)], c_int),
         "pdf_batch_enable_kernel_timing": ([vp, c_int], c_int),
         "pdf_device_copy": ([vp, vp, ctypes.c_size_t], c_int),
+        "pdf_check_exact_sequence": ([c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_uint32)], c_int),
         "pdf_set_tb_shape": ([vp, c_int], c_int),
         "pdf_batch_set_tb_shape": ([vp, c_int], c_int),
         "pdf_batch_autotune": ([vp, ctypes.c_char_p, ctypes.POINTER(c_int), ctypes.POINTER(c_int), fp], c_int),
@@ -150,6 +152,20 @@ def device_count():
     n = ctypes.c_int(0)
     hip_library().pdf_device_count(ctypes.byref(n))
     return n.value
+
+
+EXACT_SQRT, EXACT_RCP, EXACT_INV_SQRT = 0, 1, 2
+
+
+def check_exact_sequence(which, lo, hi):
+    """Runs one of the iteration kernels' short exact sequences and the compiler's IEEE operation
+    on every float whose bit pattern lies in [lo, hi]; returns (mismatches, first mismatching bit
+    pattern or None)."""
+    n, first = ctypes.c_uint64(0), ctypes.c_uint32(0)
+    st = hip_library().pdf_check_exact_sequence(which, lo, hi, ctypes.byref(n), ctypes.byref(first))
+    if st != 0:
+        raise PDFlowError(st, "pdf_check_exact_sequence")
+    return n.value, (None if n.value == 0 else first.value)
 
 
 def default_params(rows, cam_mode):
