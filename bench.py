@@ -48,16 +48,31 @@ def cpu_baseline(rows, cam_mode, seed, budget_s):
     threads = min(16, os.cpu_count() or 1)
     I1, Z1, I2, Z2 = synth_pair_float(seed, 480 // cam_mode, 640 // cam_mode)
     o = Oracle(rows, cam_mode, threads=threads)
-    n, t0 = 0, time.perf_counter()
-    while True:
-        o.solve_pair(I1, Z1, I2, Z2)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or n >= 200:
-            break
+
+    def timed(budget):
+        n, t0 = 0, time.perf_counter()
+        while True:
+            o.solve_pair(I1, Z1, I2, Z2)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= budget or n >= 200:
+                return n, el
+
+    n, el = timed(budget_s)
+    o.set_threads(1)
+    n1, el1 = timed(budget_s / 2)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
     return {"value": n / el, "unit": "pairs/s", "cores": threads, "kind": "port",
             "sample": f"{n} pairs of {480 // cam_mode}x{640 // cam_mode} (rows={rows}) in {el:.1f} s, "
-                      f"C oracle -O3 OpenMP over pixels, {threads} threads"}
+                      f"C oracle -O3 OpenMP over pixels, {threads} threads",
+            "value_1_thread": n1 / el1,
+            "sample_1_thread": f"{n1} pairs in {el1:.1f} s on 1 thread",
+            "cpu_model": model, "host_cpus": os.cpu_count()}
 
 
 def main():
