@@ -88,6 +88,9 @@ def main():
     ap.add_argument("--tb-k", type=int, default=None,
                     help="iterations per launch of the temporally blocked kernel (1 = off)")
     ap.add_argument("--tb-min", type=int, default=None, help="smallest level (pixels) it runs on")
+    ap.add_argument("--tb-shape", type=int, default=None,
+                    help="batched/learned: region shape of the temporally blocked kernel (0: 64x32, "
+                         "1: 32x32)")
     ap.add_argument("--workload", choices=("plain", "batched", "mixed", "stream", "learned"), default="learned",
                     help="plain: one pair per step; batched: --batch-pairs pairs of --rows per "
                          "step; mixed: 128 pairs (50%% 480x640, 30%% 240x320, 20%% 120x160); "
@@ -285,6 +288,8 @@ def run_batched(args):
     if args.tb_k is not None or args.tb_min is not None:
         b.set_iteration_kernel(args.tb_k if args.tb_k is not None else 4,
                                args.tb_min if args.tb_min is not None else 4800)
+    if args.tb_shape is not None:
+        b.set_tb_shape(args.tb_shape)
     distinct = {}
     for i, r in enumerate(rows_list):
         key = (r, i % 16)
@@ -380,6 +385,8 @@ def run_learned(args):
 
     def make(iters=None):
         b = Batch([rows] * n, iters=None if iters is None else [iters] * n)
+        if args.tb_shape is not None:
+            b.set_tb_shape(args.tb_shape)
         for i in range(n):
             b.set_inputs(i, *(p[i] for p in planes))
         return b

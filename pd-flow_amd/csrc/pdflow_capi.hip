@@ -840,7 +840,7 @@ int pdf_check_exact_sequence(int which, uint32_t lo, uint32_t hi, uint64_t *mism
         pdf::check_exact(which, lo, hi, (unsigned long long *)d, (unsigned *)((char *)d + 8)) != hipSuccess ||
         hipMemcpy(out, d, 16, hipMemcpyDeviceToHost) != hipSuccess)
         st = PDF_ERR_HIP;
-    hipFree(d);
+    (void)hipFree(d);
     if (st == PDF_OK) {
         *mismatches = (uint64_t)out[0] | ((uint64_t)out[1] << 32);
         *first_mismatch = out[2];

@@ -193,24 +193,50 @@ struct Px {
     bool on;
 };
 
-__device__ __forceinline__ void load_px(Px &q, const PDArgs &a, int gv, int gu, bool on)
+// Loads are unconditional: pixels outside the image load pixel 0's values, which no pixel
+// inside the image reads (every image-border term is selected away by gv/gu).
+__device__ __forceinline__ unsigned px_index(const PDArgs &a, int gv, int gu, bool on)
 {
-    // loads are unconditional: pixels outside the image load pixel 0's values, which no pixel
-    // inside the image reads (every image-border term is selected away by gv/gu)
+    return on ? (unsigned)gv + (unsigned)gu * a.rows : 0u;
+}
+
+// the 13 state planes, ri and rj
+__device__ __forceinline__ void load_state(Px &q, const PDArgs &a, int gv, int gu, bool on)
+{
     q.gv = gv;
     q.gu = gu;
     q.on = on;
-    const unsigned g = on ? (unsigned)gv + (unsigned)gu * a.rows : 0u;
-    auto ld = [&](const float *plane, float) { return plane[g]; };
-    q.du = ld(a.in[S_DU], 0.f); q.dv = ld(a.in[S_DV], 0.f); q.dw = ld(a.in[S_DW], 0.f);
-    q.ua = ld(a.in[S_DU_ACC], 0.f); q.va = ld(a.in[S_DV_ACC], 0.f); q.wa = ld(a.in[S_DW_ACC], 0.f);
-    q.pd = ld(a.in[S_PD], 0.f);
+    const unsigned g = px_index(a, gv, gu, on);
+    q.du = a.in[S_DU][g]; q.dv = a.in[S_DV][g]; q.dw = a.in[S_DW][g];
+    q.ua = a.in[S_DU_ACC][g]; q.va = a.in[S_DV_ACC][g]; q.wa = a.in[S_DW_ACC][g];
+    q.pd = a.in[S_PD][g];
 #pragma unroll
-    for (int k = 0; k < 6; k++) q.p[k] = ld(a.in[S_PUU + k], 0.f);
-    q.up = ld(a.du_prev, 0.f); q.vp = ld(a.dv_prev, 0.f); q.mu = ld(a.mu_uv, 0.f);
-    q.ddu = ld(a.ddu, 0.f); q.ddv = ld(a.ddv, 0.f); q.ddt = ld(a.ddt, 0.f);
-    q.dct = ld(a.dct, 0.f); q.dcu = ld(a.dcu, 0.f); q.dcv = ld(a.dcv, 0.f);
-    q.ri = ld(a.ri, 1.f); q.rj = ld(a.rj, 1.f);
+    for (int k = 0; k < 6; k++) q.p[k] = a.in[S_PUU + k][g];
+    q.ri = a.ri[g]; q.rj = a.rj[g];
+}
+
+// the other 9 per-level constant planes
+constexpr int kConstPlanes = 9;
+__device__ __forceinline__ const float *const_plane(const PDArgs &a, int k)
+{
+    const float *const planes[kConstPlanes] = {a.du_prev, a.dv_prev, a.mu_uv, a.ddu, a.ddv,
+                                               a.ddt, a.dct, a.dcu, a.dcv};
+    return planes[k];
+}
+__device__ __forceinline__ void set_consts(Px &q, const float (&c)[kConstPlanes])
+{
+    q.up = c[0]; q.vp = c[1]; q.mu = c[2]; q.ddu = c[3]; q.ddv = c[4];
+    q.ddt = c[5]; q.dct = c[6]; q.dcu = c[7]; q.dcv = c[8];
+}
+
+__device__ __forceinline__ void load_px(Px &q, const PDArgs &a, int gv, int gu, bool on)
+{
+    load_state(q, a, gv, gu, on);
+    const unsigned g = px_index(a, gv, gu, on);
+    float c[kConstPlanes];
+#pragma unroll
+    for (int k = 0; k < kConstPlanes; k++) c[k] = const_plane(a, k)[g];
+    set_consts(q, c);
 }
 
 __device__ __forceinline__ void store_px(const Px &q, const PDArgs &a)
@@ -336,49 +362,54 @@ __device__ __forceinline__ int tb_wave_reach(int w)
 // applies f(pixel, j) to a thread's two pixels (named variables, so they stay in registers)
 #define PD_BOTH(f) do { f(x0, 0); f(x1, 1); } while (0)
 
+// neighbour exchange of one region (11 planes)
+template <int RV, int RU>
+struct TBLds {
+    float u[RU * RV], v[RU * RV], w[RU * RV];
+    float p[6][RU * RV];
+    float ri[RU * RV], rj[RU * RV];
+};
+
+// where a thread sits in its region: lanes of a wave run down RV rows of 64/RV column pairs
+template <int RV, int RU>
+struct TBMap {
+    int lv, slot, reach;
+    __device__ __forceinline__ TBMap()
+    {
+        constexpr int S = 64 / RV, NP = RU / 2;
+        static_assert(64 % RV == 0 && RV * RU / 128 * S == NP, "waves x pairs per wave = column pairs");
+        const int lane = threadIdx.x & 63;
+        const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+        lv = lane % RV;
+        slot = tb_pair<S, NP>(wave, lane / RV);                  // this thread's column pair
+        reach = tb_wave_reach<RV, RU>(wave);                     // wave-uniform
+    }
+};
+
+// The K iterations of one region whose origin (rv0, ru0) is K rows and columns before its
+// output tile: both pixels are loaded and their ri, rj are in s.ri / s.rj behind a barrier.
+// Derives the step sizes, iterates, and writes back the inner tile.
 template <int K, int RV, int RU>
-__device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned by)
+__device__ __forceinline__ void tb_iterate(const PDArgs &a, TBLds<RV, RU> &s, const TBMap<RV, RU> &m,
+                                           Px &x0, Px &x1)
 {
     constexpr int TVk = RV - 2 * K, TUk = RU - 2 * K;
-    static_assert(64 % RV == 0 && TVk > 0 && TUk > 0, "region shape");
-    __shared__ float sU[RU * RV], sV[RU * RV], sW[RU * RV];
-    __shared__ float sP[6][RU * RV];
-    __shared__ float sRi[RU * RV], sRj[RU * RV];
-
-    constexpr int S = 64 / RV, NP = RU / 2;
-    static_assert(RV * RU / 128 * S == NP, "waves x pairs per wave = column pairs");
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int rv0 = (int)bx * TVk - K, ru0 = (int)by * TUk - K;  // region origin
     const int rows = (int)a.rows, cols = (int)a.cols;
     const float li = a.lambda_i, ld = a.lambda_d;
-    const int lv = lane % RV;
-    const int slot = tb_pair<S, NP>(wave, lane / RV);            // this thread's column pair
-    const int reach = tb_wave_reach<RV, RU>(wave);               // wave-uniform
-
-    Px x0, x1;
-    auto load = [&](Px &q, int j) __attribute__((always_inline)) {
-        const int lu = 2 * slot + j;
-        const int gv = rv0 + lv, gu = ru0 + lu;
-        load_px(q, a, gv, gu, gv >= 0 && gv < rows && gu >= 0 && gu < cols);
-        sRi[lu * RV + lv] = q.ri;
-        sRj[lu * RV + lv] = q.rj;
-    };
-    PD_BOTH(load);
-    __syncthreads();
+    const int lv = m.lv, slot = m.slot, reach = m.reach;
     // neighbours outside the region only feed pixels of the discarded band, so their
     // indices are clamped into the region
     auto derive = [&](Px &q, int j) __attribute__((always_inline)) {
         const int lu = 2 * slot + j, idx = lu * RV + lv;
-        derive_px(q, sRi[lu > 0 ? idx - RV : idx], sRj[lv > 0 ? idx - 1 : idx], li, ld);
+        derive_px(q, s.ri[lu > 0 ? idx - RV : idx], s.rj[lv > 0 ? idx - 1 : idx], li, ld);
     };
     PD_BOTH(derive);
 
     auto put_bar = [&](Px &q, int j) __attribute__((always_inline)) {
         const int idx = (2 * slot + j) * RV + lv;
-        sU[idx] = q.ua + q.up;
-        sV[idx] = q.va + q.vp;
-        sW[idx] = q.wa;
+        s.u[idx] = q.ua + q.up;
+        s.v[idx] = q.va + q.vp;
+        s.w[idx] = q.wa;
     };
     // pixels outside the image run the same arithmetic on the values loaded for them; no pixel
     // inside reads their results (the image-border terms are selected away), and without a
@@ -387,16 +418,16 @@ __device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned b
         const int lu = 2 * slot + j, idx = lu * RV + lv;
         const int iu = lu + 1 < RU ? idx + RV : idx;
         const int iv = lv + 1 < RV ? idx + 1 : idx;
-        dual_px(q, rows, cols, sU[iu], sV[iu], sW[iu], sU[iv], sV[iv], sW[iv]);
+        dual_px(q, rows, cols, s.u[iu], s.v[iu], s.w[iu], s.u[iv], s.v[iv], s.w[iv]);
 #pragma unroll
-        for (int k = 0; k < 6; k++) sP[k][idx] = q.p[k];
+        for (int k = 0; k < 6; k++) s.p[k][idx] = q.p[k];
     };
     auto primal = [&](Px &q, int j) __attribute__((always_inline)) {
         const int lu = 2 * slot + j, idx = lu * RV + lv;
         const int ju = lu > 0 ? idx - RV : idx;
         const int jv = lv > 0 ? idx - 1 : idx;
-        primal_px(q, rows, cols, sRi[ju], sP[0][ju], sP[2][ju], sP[4][ju], sRj[jv], sP[1][jv],
-                  sP[3][jv], sP[5][jv], li, ld);
+        primal_px(q, rows, cols, s.ri[ju], s.p[0][ju], s.p[2][ju], s.p[4][ju], s.rj[jv], s.p[1][jv],
+                  s.p[3][jv], s.p[5][jv], li, ld);
     };
     for (int it = 0; it < K; it++) {
         if (reach >= it) PD_BOTH(put_bar);
@@ -412,6 +443,29 @@ __device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned b
         if (q.on && lv >= K && lv < K + TVk && lu >= K && lu < K + TUk) store_px(q, a);
     };
     PD_BOTH(store);
+}
+
+template <int K, int RV, int RU>
+__device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned by)
+{
+    constexpr int TVk = RV - 2 * K, TUk = RU - 2 * K;
+    static_assert(TVk > 0 && TUk > 0, "region shape");
+    __shared__ TBLds<RV, RU> s;
+    const TBMap<RV, RU> m;
+    const int rv0 = (int)bx * TVk - K, ru0 = (int)by * TUk - K;  // region origin
+    const int rows = (int)a.rows, cols = (int)a.cols;
+
+    Px x0, x1;
+    auto load = [&](Px &q, int j) __attribute__((always_inline)) {
+        const int lu = 2 * m.slot + j;
+        const int gv = rv0 + m.lv, gu = ru0 + lu;
+        load_px(q, a, gv, gu, gv >= 0 && gv < rows && gu >= 0 && gu < cols);
+        s.ri[lu * RV + m.lv] = q.ri;
+        s.rj[lu * RV + m.lv] = q.rj;
+    };
+    PD_BOTH(load);
+    __syncthreads();
+    tb_iterate<K, RV, RU>(a, s, m, x0, x1);
 }
 
 // ------------------------------------------------------------------ whole level per launch

@@ -21,7 +21,8 @@ def single(rows, seed, iters=None):
     return (I1, Z1, I2, Z2), g.solve_pair(I1, Z1, I2, Z2, iters=iters)
 
 
-@pytest.mark.parametrize("k,graph,shape", [(4, True, 0), (2, False, 0), (0, True, 0), (4, True, 1), (3, False, 1)])
+@pytest.mark.parametrize("k,graph,shape", [(4, True, 0), (2, False, 0), (0, True, 0), (4, True, 1), (3, False, 1),
+                                           (5, True, 1), (6, False, 1)])
 def test_mixed_batch_matches_single_pairs(gpu_available, k, graph, shape):
     rows = [480, 240, 120, 60, 240, 30, 120, 15, 480]
     say(f"mixed batch of {len(rows)} pairs, k={k}, graph={graph}, shape={shape}")
