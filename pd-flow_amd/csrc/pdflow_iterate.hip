@@ -321,11 +321,13 @@ __device__ __forceinline__ void primal_px(Px &q, int rows, int cols, float riu, 
 }
 
 // ------------------------------------------------------------------ K iterations per launch
-// Region shapes (rows RV x columns RU, RV*RU/2 threads, two pixels per thread in adjacent
-// columns): lanes of a wave run down RV consecutive rows of 64/RV column pairs.
-//   shape 0: 64 x 32, 1024 threads (one workgroup per CU: 88 KB LDS, ~118 VGPRs)
-//   shape 1: 32 x 32,  512 threads (two workgroups per CU, so one's loads overlap the other's
-//            arithmetic; more halo per output pixel)
+// Region shapes (rows RV x columns RU, 1024 threads; lanes of a wave run down RV consecutive
+// rows of one or two column pairs):
+//   shape 0: 64 x 32, two pixels per thread in adjacent columns, a wave = one column pair
+//            (one workgroup per CU: 88 KB LDS, ~120 VGPRs)
+//   shape 1: 32 x 32, one pixel per thread, a wave = the two columns of one pair (two
+//            workgroups per CU, 45 KB LDS each, ~62 VGPRs, so 8 waves per SIMD hide the
+//            latency of the arithmetic and one workgroup's loads overlap the other's work)
 //
 // Work that no output needs is skipped a whole wave at a time.  With d a pixel's distance to
 // the region's nearest edge (in rows or columns), iteration `it` (0-based) needs the dual step
@@ -337,30 +339,36 @@ __device__ __forceinline__ void primal_px(Px &q, int rows, int cols, float riu, 
 // small d and drop out after their first iterations.
 constexpr int BT = 1024;               // threads of the whole-level kernel
 
-// column pair (columns 2q, 2q+1 of the region) of slot s of wave w; S pairs per wave
-template <int S, int NP>
-__device__ __forceinline__ int tb_pair(int w, int s)
+// threads of a temporally blocked workgroup, and pixels per thread of a region shape
+constexpr int kTBThreads = 1024;
+template <int RV, int RU>
+constexpr int tb_px() { return RV * RU / kTBThreads; }
+// waves per SIMD the register budget must allow: two 16-wave workgroups per CU with one pixel
+// per thread (at most 64 VGPRs), one workgroup with two
+template <int RV, int RU>
+constexpr int tb_waves_per_simd() { return tb_px<RV, RU>() == 1 ? 8 : 4; }
+
+// column pair (columns 2q, 2q+1 of the region) of wave w: pairs are dealt from both ends
+// inwards (0, NP-1, 1, NP-2, ...), one pair per wave
+template <int NP>
+__device__ __forceinline__ int tb_pair(int w)
 {
-    static_assert(S == 1 || S == 2, "one or two column pairs per wave");
-    return (S == 1 || s == 0) ? w : NP - 1 - w;
+    return (w & 1) ? NP - 1 - (w >> 1) : (w >> 1);
 }
 
-// largest distance to the region edge over the pixels of wave w (it spans all RV rows)
+// largest distance to the region edge over the pixels of a wave that holds column pair q and
+// spans all RV rows
 template <int RV, int RU>
-__device__ __forceinline__ int tb_wave_reach(int w)
+__device__ __forceinline__ int tb_pair_reach(int q)
 {
-    constexpr int S = 64 / RV, NP = RU / 2;
-    int reach = 0;
-#pragma unroll
-    for (int s = 0; s < S; s++) {
-        const int c = 2 * tb_pair<S, NP>(w, s);
-        reach = max(reach, max(min(c, RU - 1 - c), min(c + 1, RU - 2 - c)));
-    }
-    return min(reach, (RV - 1) / 2);
+    const int c = 2 * q;
+    return min(max(min(c, RU - 1 - c), min(c + 1, RU - 2 - c)), (RV - 1) / 2);
 }
 
 // applies f(pixel, j) to a thread's two pixels (named variables, so they stay in registers)
 #define PD_BOTH(f) do { f(x0, 0); f(x1, 1); } while (0)
+// applies f(pixel, j) to a thread's pixels under a region map m: column 2 * m.slot + j
+#define PD_EACH(f) do { if constexpr (PX == 2) { f(x0, 0); f(x1, 1); } else { f(x0, m.j); } } while (0)
 
 // neighbour exchange of one region (11 planes)
 template <int RV, int RU>
@@ -370,19 +378,22 @@ struct TBLds {
     float ri[RU * RV], rj[RU * RV];
 };
 
-// where a thread sits in its region: lanes of a wave run down RV rows of 64/RV column pairs
+// where a thread sits in its region: a wave holds one column pair (slot) down all RV rows;
+// with two pixels per thread (RV = 64) a thread holds both columns of its row, with one
+// (RV = 32) lanes 0-31 hold column 2 * slot and lanes 32-63 column 2 * slot + 1 (j)
 template <int RV, int RU>
 struct TBMap {
-    int lv, slot, reach;
+    int lv, slot, reach, j;
     __device__ __forceinline__ TBMap()
     {
-        constexpr int S = 64 / RV, NP = RU / 2;
-        static_assert(64 % RV == 0 && RV * RU / 128 * S == NP, "waves x pairs per wave = column pairs");
+        constexpr int PX = tb_px<RV, RU>(), NP = RU / 2;
+        static_assert(PX * 64 == 2 * RV && RV * RU / PX / 64 == NP, "one column pair per wave");
         const int lane = threadIdx.x & 63;
         const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
         lv = lane % RV;
-        slot = tb_pair<S, NP>(wave, lane / RV);                  // this thread's column pair
-        reach = tb_wave_reach<RV, RU>(wave);                     // wave-uniform
+        j = PX == 1 ? lane / RV : 0;
+        slot = tb_pair<NP>(wave);
+        reach = tb_pair_reach<RV, RU>(slot);                     // wave-uniform
     }
 };
 
@@ -393,7 +404,7 @@ template <int K, int RV, int RU>
 __device__ __forceinline__ void tb_iterate(const PDArgs &a, TBLds<RV, RU> &s, const TBMap<RV, RU> &m,
                                            Px &x0, Px &x1)
 {
-    constexpr int TVk = RV - 2 * K, TUk = RU - 2 * K;
+    constexpr int TVk = RV - 2 * K, TUk = RU - 2 * K, PX = tb_px<RV, RU>();
     const int rows = (int)a.rows, cols = (int)a.cols;
     const float li = a.lambda_i, ld = a.lambda_d;
     const int lv = m.lv, slot = m.slot, reach = m.reach;
@@ -403,7 +414,7 @@ __device__ __forceinline__ void tb_iterate(const PDArgs &a, TBLds<RV, RU> &s, co
         const int lu = 2 * slot + j, idx = lu * RV + lv;
         derive_px(q, s.ri[lu > 0 ? idx - RV : idx], s.rj[lv > 0 ? idx - 1 : idx], li, ld);
     };
-    PD_BOTH(derive);
+    PD_EACH(derive);
 
     auto put_bar = [&](Px &q, int j) __attribute__((always_inline)) {
         const int idx = (2 * slot + j) * RV + lv;
@@ -430,11 +441,11 @@ __device__ __forceinline__ void tb_iterate(const PDArgs &a, TBLds<RV, RU> &s, co
                   s.p[3][jv], s.p[5][jv], li, ld);
     };
     for (int it = 0; it < K; it++) {
-        if (reach >= it) PD_BOTH(put_bar);
+        if (reach >= it) PD_EACH(put_bar);
         __syncthreads();
-        if (reach >= it) PD_BOTH(dual);
+        if (reach >= it) PD_EACH(dual);
         __syncthreads();
-        if (reach > it) PD_BOTH(primal);
+        if (reach > it) PD_EACH(primal);
     }
 
     // write back the inner tile
@@ -442,13 +453,13 @@ __device__ __forceinline__ void tb_iterate(const PDArgs &a, TBLds<RV, RU> &s, co
         const int lu = 2 * slot + j;
         if (q.on && lv >= K && lv < K + TVk && lu >= K && lu < K + TUk) store_px(q, a);
     };
-    PD_BOTH(store);
+    PD_EACH(store);
 }
 
 template <int K, int RV, int RU>
 __device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned by)
 {
-    constexpr int TVk = RV - 2 * K, TUk = RU - 2 * K;
+    constexpr int TVk = RV - 2 * K, TUk = RU - 2 * K, PX = tb_px<RV, RU>();
     static_assert(TVk > 0 && TUk > 0, "region shape");
     __shared__ TBLds<RV, RU> s;
     const TBMap<RV, RU> m;
@@ -463,7 +474,7 @@ __device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned b
         s.ri[lu * RV + m.lv] = q.ri;
         s.rj[lu * RV + m.lv] = q.rj;
     };
-    PD_BOTH(load);
+    PD_EACH(load);
     __syncthreads();
     tb_iterate<K, RV, RU>(a, s, m, x0, x1);
 }
@@ -544,7 +555,8 @@ __device__ __forceinline__ void level_body(const PDArgs &a, int n)
 // ------------------------------------------------------------------ entry points
 __global__ void __launch_bounds__(TV * TU) k_pd_fused(PDArgs a) { pd_fused_body(a, blockIdx.x, blockIdx.y); }
 template <int K, int RV, int RU>
-__global__ void __launch_bounds__(RV * RU / 2) k_pd_tb(PDArgs a)
+__global__ void __launch_bounds__(kTBThreads) __attribute__((amdgpu_waves_per_eu(tb_waves_per_simd<RV, RU>())))
+k_pd_tb(PDArgs a)
 {
     // tiles numbered v-major, visited in XCD-contiguous runs
     const unsigned t = xcd_swizzle(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
@@ -561,7 +573,8 @@ __global__ void __launch_bounds__(TV * TU) kb_pd_fused(BatchTable t)
     pd_fused_body(a, it.block % tiles_v, it.block / tiles_v);
 }
 template <int K, int RV, int RU>
-__global__ void __launch_bounds__(RV * RU / 2) kb_pd_tb(BatchTable t)
+__global__ void __launch_bounds__(kTBThreads) __attribute__((amdgpu_waves_per_eu(tb_waves_per_simd<RV, RU>())))
+kb_pd_tb(BatchTable t)
 {
     const BatchItem it = batch_item(t, xcd_swizzle(blockIdx.x, gridDim.x));
     const PDArgs &a = static_cast<const PDArgs *>(t.args)[it.item];
@@ -610,7 +623,7 @@ static hipError_t launch_tb(const PDArgs &a, hipStream_t s)
 {
     unsigned tv, tu;
     tb_region<RV, RU>(K, a.rows, a.cols, &tv, &tu);
-    hipLaunchKernelGGL((k_pd_tb<K, RV, RU>), dim3(tv, tu), dim3(RV * RU / 2), 0, s, a);
+    hipLaunchKernelGGL((k_pd_tb<K, RV, RU>), dim3(tv, tu), dim3(kTBThreads), 0, s, a);
     return hipGetLastError();
 }
 
@@ -655,7 +668,7 @@ unsigned pd_blocks(int k, int shape, unsigned rows, unsigned cols)
 template <int RV, int RU>
 static hipError_t launch_tb_batch(int k, const BatchTable &t, const dim3 &grid, hipStream_t s)
 {
-    const dim3 block(RV * RU / 2);
+    const dim3 block(kTBThreads);
     switch (k) {
     case 2: hipLaunchKernelGGL((kb_pd_tb<2, RV, RU>), grid, block, 0, s, t); break;
     case 3: hipLaunchKernelGGL((kb_pd_tb<3, RV, RU>), grid, block, 0, s, t); break;
