@@ -2,7 +2,8 @@
 
 Default workload (--workload learned): 256 VGA (480x640) RGB-D pairs per step, batched with the
 learned warm start -- both pyramids of every pair, one network pass over their 60x80 levels,
-the solves from there with every level's iterations divided by --factor (2), then the fallback rule (final
+the solves from there with every level's iterations divided by --factor (2), 25 more finest-level
+iterations (--continue-iters) for the pairs that miss the rule, then the fallback rule (final
 energy or warped residual more than 5% above the plain solver's) and the plain re-solve of the
 pairs it rejects, all inside the timed region.  The line also carries the plain single-pair and
 plain batched rates of the same pairs, end-point errors against the plain solve and against the
@@ -107,6 +108,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run a plain device copy of BYTES three times (the calibration point "
                          "of HBM byte counters under rocprofv3 --pmc)")
+    ap.add_argument("--continue-iters", type=int, default=25, metavar="N",
+                    help="learned: pairs that miss the fallback rule first get N more finest-level "
+                         "iterations from where they stand (pdf_batch_continue); the rule then "
+                         "re-solves those that still miss it")
     ap.add_argument("--fallback", choices=("energy_residual", "residual", "none"), default="energy_residual",
                     help="learned: rule deciding which pairs the plain path re-solves (tolerance 5%%)")
     args = ap.parse_args()
@@ -405,9 +410,19 @@ def run_learned(args):
     e_ref = {"energy_residual": pe, "residual": inf, "none": inf}[args.fallback]
     r_ref = {"energy_residual": pr, "residual": pr, "none": inf}[args.fallback]
 
+    def finish():
+        """the rule after the learned solve: optional continuation of the pairs that miss it,
+        then the plain re-solve of those that still do; returns (continued, fell back) flags"""
+        cont = np.zeros(n, np.uint8)
+        if args.continue_iters > 0:
+            e, r = learned.metrics()
+            cont = (~(e <= 1.05 * e_ref) | ~(r <= 1.05 * r_ref)).astype(np.uint8)
+            learned.continue_finest(np.nonzero(cont)[0], args.continue_iters)
+        return cont, learned.fallback(e_ref, r_ref, 0.05)
+
     def step():
         learned.solve(graph=not args.no_graph)
-        return learned.fallback(e_ref, r_ref, 0.05)
+        return finish()[1]
 
     for _ in range(max(1, args.warmup)):
         flags = step()
@@ -448,7 +463,7 @@ def run_learned(args):
     epe_l = learned.epe(plain)
     le, lr_ = learned.metrics()
     gt_learned = gt_epe(learned)
-    flags = learned.fallback(e_ref, r_ref, 0.05)
+    cont, flags = finish()
     epe_f = learned.epe(plain)
     ablation.solve(graph=True)
     ablation.synchronize()
@@ -482,7 +497,9 @@ def run_learned(args):
         "data": f"synthetic (seeded RGB-D generator, seeds {args.seed}..{args.seed + n - 1})",
         "config": {"workload": f"batched + learned warm start, {n} pairs of {rows}x{rows * 4 // 3}: "
                                f"network at the 60x80 level, iterations {cut} (reference / {args.factor}), "
-                               f"fallback rule '{args.fallback}' (5%) with plain re-solves inside the timed "
+                               + (f"pairs missing the rule continued by {args.continue_iters} finest-level "
+                                  "iterations, " if args.continue_iters else "")
+                               + f"fallback rule '{args.fallback}' (5%) with plain re-solves inside the timed "
                                "region" + (", hipGraph" if not args.no_graph else ""),
                    "pairs_per_step": n, "weights": os.path.relpath(args.weights, ROOT)},
         "vs_plain": {"plain_single_pair_pairs_s": round(plain_single, 3),
@@ -495,6 +512,8 @@ def run_learned(args):
                       "epe_vs_truth_learned": stats(gt_learned), "epe_vs_truth_after_fallback": stats(gt_final),
                       "epe_vs_truth_ablation": stats(gt_abl),
                       "energy_ratio_learned": stats(le / pe), "residual_ratio_learned": stats(lr_ / pr),
+                      "continued_fraction": round(float(cont.mean()), 4),
+                      "continue_iters": args.continue_iters,
                       "fallback_fraction": round(float(flags.mean()), 4)},
         "roofline": {"bound": "hbm", "kernel": "kb_pd_tb<K, 32, 32> (batched, K = 2 to 5 primal-dual iterations per launch)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -266,6 +266,12 @@ int pdf_batch_epe(pdf_batch *b, const pdf_batch *ref, float *epe);
 /* re-solves the listed pairs in place with the plain path: level 0, the reference schedule for
  * the pair's level count (whatever pdf_batch_set_schedule set), the pyramids already built */
 int pdf_batch_resolve_plain(pdf_batch *b, const unsigned *pairs, unsigned count);
+/* No counterpart in the reference: runs extra_iters more primal-dual iterations of the listed
+ * pairs (each at most once) at their finest level, continuing from the state the last solve,
+ * continuation or re-solve left, then that level's median filter and motion field again.  The
+ * flow equals that of a solve whose finest level ran its iterations plus extra_iters.  For the
+ * learned path's pairs that miss the fallback rule by little (an adaptive iteration budget). */
+int pdf_batch_continue(pdf_batch *b, const unsigned *pairs, unsigned count, unsigned extra_iters);
 /* fallback rule: every pair whose energy or residual exceeds (1 + tol) times the reference
  * value (e.g. the plain solver's, from pdf_batch_metrics of a plain batch) is re-solved by
  * the plain path; flags[i] = 1 for those (flags may be NULL), *count their number */

@@ -21,6 +21,7 @@
 // 0, the reference schedule) from the pyramids already built.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -264,6 +265,61 @@ pdf::LearnArgs learn_args(const pdf_batch *b, const Pair &p)
     return a;
 }
 
+// Records the iteration launches of the active pairs at their current level (rows_i, cols_i
+// set), n_of(p) iterations each.  Each pair's launches (iterations per launch, 0 = whole level)
+// are consumed in rounds; one launch per round and launch size covers every pair whose next
+// launch has that size.
+template <class N>
+void add_iterations(pdf_batch *b, Plan &plan, const std::vector<unsigned> &active, N n_of)
+{
+    const unsigned P = active.empty() ? 0 : b->pairs[active[0]].rows_i * b->pairs[active[0]].cols_i;
+    const bool whole = b->level_kernel && pdf::pd_level_supported(P);
+    const int K = (b->tb_k > 1 && P >= b->tb_min_pixels) ? b->tb_k : 1;
+    std::vector<std::vector<int>> todo(active.size());
+    for (size_t j = 0; j < active.size(); j++) {
+        const unsigned n = n_of(b->pairs[active[j]]);
+        if (whole) { if (n) todo[j].push_back(0); }
+        else todo[j] = pdf::tb_chunks(n, K);
+    }
+    std::vector<size_t> pos(active.size(), 0);
+    for (;;) {
+        int k = -1;
+        for (size_t j = 0; j < active.size() && k < 0; j++)
+            if (pos[j] < todo[j].size()) k = todo[j][pos[j]];
+        if (k < 0) break;
+        std::vector<pdf::PDArgs> args;
+        std::vector<unsigned> blocks;
+        uint64_t pix = 0;
+        for (size_t j = 0; j < active.size(); j++) {
+            if (pos[j] >= todo[j].size() || todo[j][pos[j]] != k) continue;
+            Pair &p = b->pairs[active[j]];
+            const unsigned n = k == 0 ? n_of(p) : (unsigned)k;
+            args.push_back(pd_args(b, p, n));
+            blocks.push_back(pdf::pd_blocks(k, b->tb_shape, p.rows_i, p.cols_i));
+            pix += (uint64_t)p.rows_i * p.cols_i * n;
+            pos[j]++;
+            swap_state(p);
+        }
+        Launch l{};
+        l.kind = Launch::PD;
+        l.k = k;
+        l.pixel_iters = pix;
+        add_launch(plan, l, args, blocks);
+    }
+}
+
+void add_setup(pdf_batch *b, Plan &plan, pdf::BatchKernel k, int flag, const std::vector<unsigned> &ids)
+{
+    std::vector<pdf::LevelArgs> args;
+    std::vector<unsigned> blocks;
+    for (unsigned i : ids) {
+        const Pair &p = b->pairs[i];
+        args.push_back(level_args(b, p));
+        blocks.push_back(pdf::batch_blocks(k, p.rows_i, p.cols_i));
+    }
+    add_launch(plan, setup_launch(k, flag), args, blocks);
+}
+
 // Simulates a solve and records its launches.  who: the pairs solved; fallback: the plain
 // path from level 0 with the reference schedule and without the pyramid (already built).
 void build_plan(pdf_batch *b, const std::vector<unsigned> &who, bool fallback, Plan &plan)
@@ -324,61 +380,40 @@ void build_plan(pdf_batch *b, const std::vector<unsigned> &who, bool fallback, P
             active.push_back(i);
             (st > 0 && s == st ? warm_now : cold_now).push_back(i);
         }
-        auto setup_for = [&](pdf::BatchKernel k, int flag, const std::vector<unsigned> &ids) {
-            std::vector<pdf::LevelArgs> args;
-            std::vector<unsigned> blocks;
-            for (unsigned i : ids) {
-                const Pair &p = b->pairs[i];
-                args.push_back(level_args(b, p));
-                blocks.push_back(pdf::batch_blocks(k, p.rows_i, p.cols_i));
-            }
-            add_launch(plan, setup_launch(k, flag), args, blocks);
-        };
-        auto setup = [&](pdf::BatchKernel k, int flag) { setup_for(k, flag, active); };
-        setup_for(pdf::BatchKernel::LevelInit, s > 0 ? 1 : 0, cold_now);
-        setup_for(pdf::BatchKernel::WarmInit, 0, warm_now);
-        setup(pdf::BatchKernel::RijGradients, 0);
-        setup(pdf::BatchKernel::WarpMu, 0);
-
-        // iterations: each active pair's launches (iterations per launch, 0 = whole level),
-        // consumed in rounds; one launch per round and launch size covers every pair whose
-        // next launch has that size
-        const unsigned P = active.empty() ? 0 : b->pairs[active[0]].rows_i * b->pairs[active[0]].cols_i;
-        const bool whole = b->level_kernel && pdf::pd_level_supported(P);
-        const int K = (b->tb_k > 1 && P >= b->tb_min_pixels) ? b->tb_k : 1;
-        std::vector<std::vector<int>> todo(active.size());
-        for (size_t j = 0; j < active.size(); j++) {
-            const unsigned n = iters_of(b->pairs[active[j]], s);
-            if (whole) { if (n) todo[j].push_back(0); }
-            else todo[j] = pdf::tb_chunks(n, K);
-        }
-        std::vector<size_t> pos(active.size(), 0);
-        for (;;) {
-            int k = -1;
-            for (size_t j = 0; j < active.size() && k < 0; j++)
-                if (pos[j] < todo[j].size()) k = todo[j][pos[j]];
-            if (k < 0) break;
-            std::vector<pdf::PDArgs> args;
-            std::vector<unsigned> blocks;
-            uint64_t pix = 0;
-            for (size_t j = 0; j < active.size(); j++) {
-                if (pos[j] >= todo[j].size() || todo[j][pos[j]] != k) continue;
-                Pair &p = b->pairs[active[j]];
-                const unsigned n = k == 0 ? iters_of(p, s) : (unsigned)k;
-                args.push_back(pd_args(b, p, n));
-                blocks.push_back(pdf::pd_blocks(k, b->tb_shape, p.rows_i, p.cols_i));
-                pix += (uint64_t)p.rows_i * p.cols_i * n;
-                pos[j]++;
-                swap_state(p);
-            }
-            Launch l{};
-            l.kind = Launch::PD;
-            l.k = k;
-            l.pixel_iters = pix;
-            add_launch(plan, l, args, blocks);
-        }
-        setup(pdf::BatchKernel::FilterMotion, 0);
+        add_setup(b, plan, pdf::BatchKernel::LevelInit, s > 0 ? 1 : 0, cold_now);
+        add_setup(b, plan, pdf::BatchKernel::WarmInit, 0, warm_now);
+        add_setup(b, plan, pdf::BatchKernel::RijGradients, 0, active);
+        add_setup(b, plan, pdf::BatchKernel::WarpMu, 0, active);
+        add_iterations(b, plan, active, [&](const Pair &p) { return iters_of(p, s); });
+        add_setup(b, plan, pdf::BatchKernel::FilterMotion, 0, active);
     }
+}
+
+// Records `extra` more iterations of the listed pairs at their finest level, continuing from
+// the state they are in (the level's constants are still in place), then the median filter
+// and motion field of that level again.  The filter only reads the primal-dual state, so the
+// result equals a solve whose finest level ran its iterations plus `extra` in one go.
+void build_continue_plan(pdf_batch *b, const std::vector<unsigned> &who, unsigned extra, Plan &plan)
+{
+    plan.launches.clear();
+    plan.tables.clear();
+    for (unsigned i : who) {
+        Pair &p = b->pairs[i];
+        level_dims(p, p.ctf_levels - 1, &p.rows_i, &p.cols_i, &p.level_image);
+    }
+    // one iteration schedule per level size: pairs of different sizes go in separate rounds
+    std::vector<unsigned> sizes;
+    for (unsigned i : who) {
+        const unsigned P = b->pairs[i].rows_i * b->pairs[i].cols_i;
+        if (std::find(sizes.begin(), sizes.end(), P) == sizes.end()) sizes.push_back(P);
+    }
+    for (unsigned P : sizes) {
+        std::vector<unsigned> same;
+        for (unsigned i : who)
+            if (b->pairs[i].rows_i * b->pairs[i].cols_i == P) same.push_back(i);
+        add_iterations(b, plan, same, [&](const Pair &) { return extra; });
+    }
+    add_setup(b, plan, pdf::BatchKernel::FilterMotion, 0, who);
 }
 
 void canonicalize(Pair &p)
@@ -955,6 +990,24 @@ int pdf_batch_resolve_plain(pdf_batch *b, const unsigned *pairs, unsigned count)
         if (i >= b->pairs.size()) return bfail(b, PDF_ERR_ARG, "bad pair index");
     for (unsigned i : who) canonicalize(b->pairs[i]);
     build_plan(b, who, true, b->fb);
+    char *tables = nullptr;
+    int rc = upload_scratch_plan(b, b->fb, 0, &tables, nullptr);
+    if (rc) return rc;
+    return run_plan(b, b->fb, tables, false);
+}
+
+int pdf_batch_continue(pdf_batch *b, const unsigned *pairs, unsigned count, unsigned extra_iters)
+{
+    if (!b || (count && !pairs)) return PDF_ERR_ARG;
+    if (!b->planned) return bfail(b, PDF_ERR_STATE, "continue before any solve");
+    if (extra_iters > kMaxIters) return bfail(b, PDF_ERR_ARG, "iteration count too large");
+    if (count == 0 || extra_iters == 0) return PDF_OK;
+    std::vector<unsigned> who(pairs, pairs + count);
+    for (unsigned i : who)
+        if (i >= b->pairs.size()) return bfail(b, PDF_ERR_ARG, "bad pair index");
+    std::sort(who.begin(), who.end());
+    if (std::adjacent_find(who.begin(), who.end()) != who.end()) return bfail(b, PDF_ERR_ARG, "pair listed twice");
+    build_continue_plan(b, who, extra_iters, b->fb);
     char *tables = nullptr;
     int rc = upload_scratch_plan(b, b->fb, 0, &tables, nullptr);
     if (rc) return rc;

@@ -120,6 +120,7 @@ def hip_library():
         "pdf_stream_frame_device": ([vp, vp, vp, up, c_int, ctypes.POINTER(c_int)], c_int),
         "pdf_batch_enable_kernel_timing": ([vp, c_int], c_int),
         "pdf_device_copy": ([vp, vp, ctypes.c_size_t], c_int),
+        "pdf_batch_continue": ([vp, ctypes.POINTER(c_uint), c_uint, c_uint], c_int),
         "pdf_check_exact_sequence": ([c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_uint32)], c_int),
         "pdf_set_tb_shape": ([vp, c_int], c_int),
@@ -521,6 +522,14 @@ class Batch:
         n = len(pairs)
         arr = (ctypes.c_uint * max(n, 1))(*pairs)
         self._chk(self.lib.pdf_batch_resolve_plain(self.ctx, arr, n))
+
+    def continue_finest(self, pairs, extra_iters):
+        """extra_iters more iterations of the listed pairs at their finest level, continuing from
+        their current state, then the level's filter and motion field (pdf_batch_continue)"""
+        pairs = [int(i) for i in pairs]
+        n = len(pairs)
+        arr = (ctypes.c_uint * max(n, 1))(*pairs)
+        self._chk(self.lib.pdf_batch_continue(self.ctx, arr, n, int(extra_iters)))
 
     def fallback(self, energy_ref, residual_ref, tol=0.05):
         """re-solves the pairs failing the rule; returns their flags (uint8 per pair)"""

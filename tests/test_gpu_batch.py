@@ -46,6 +46,48 @@ def test_mixed_batch_matches_single_pairs(gpu_available, k, graph, shape):
     say(f"  {b.launch_count()} launches per batched solve")
 
 
+def test_continue_equals_a_longer_finest_level(gpu_available):
+    """pdf_batch_continue: n more finest-level iterations from the solved state give exactly
+    the flow of a solve whose finest level ran n more; continuing twice adds up; the next
+    solve starts afresh."""
+    rows = [240, 120, 240, 60]
+    scheds = [[10, 11, 13, 17, 21], [3, 5, 7, 9], [40, 55, 70, 85, 100], [6, 9, 12]]
+    cont = [0, 2, 3]                      # pair 1 is not continued
+    say("continuation of the finest level: +7 then +5 against +12 in one solve")
+    a = Batch(rows, iters=scheds)
+    longer = [list(sc) for sc in scheds]
+    for i in cont:
+        longer[i][-1] += 12
+    b = Batch(rows, iters=longer)
+    for i, r in enumerate(rows):
+        inputs = single(r, 900 + i)[0]
+        a.set_inputs(i, *inputs)
+        b.set_inputs(i, *inputs)
+    a.solve(graph=True)
+    a.synchronize()
+    first = [a.flow(i) for i in range(len(rows))]
+    a.continue_finest(cont, 7)
+    a.continue_finest(cont, 5)
+    a.synchronize()
+    b.solve(graph=True)
+    b.synchronize()
+    for i, r in enumerate(rows):
+        for x, y in zip(a.flow(i), b.flow(i)):
+            np.testing.assert_array_equal(x, y, err_msg=f"pair {i} rows {r}")
+    ea, ra = a.metrics()
+    eb, rb = b.metrics()
+    np.testing.assert_array_equal(ea, eb)
+    np.testing.assert_array_equal(ra, rb)
+    # a new solve of batch a replays its own schedule from scratch
+    a.solve(graph=True)
+    a.synchronize()
+    for i in range(len(rows)):
+        for x, y in zip(a.flow(i), first[i]):
+            np.testing.assert_array_equal(x, y)
+    with pytest.raises(Exception):
+        a.continue_finest([0, 0], 3)
+
+
 def test_batch_with_custom_schedules(gpu_available):
     """Per-pair iteration counts (as the learned warm start uses), odd counts included."""
     rows = [240, 120, 240]
