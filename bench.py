@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run a plain device copy of BYTES three times (the calibration point "
                          "of HBM byte counters under rocprofv3 --pmc)")
+    ap.add_argument("--finest-iters", type=int, default=None, metavar="N",
+                    help="learned: the finest level's iteration count in place of the cut schedule's")
     ap.add_argument("--continue-iters", type=int, default=25, metavar="N",
                     help="learned: pairs that miss the fallback rule first get N more finest-level "
                          "iterations from where they stand (pdf_batch_continue); the rule then "
@@ -387,6 +389,8 @@ def run_learned(args):
     net = Net(max_batch=n, h=60, w=80)
     net.load(args.weights)
     cut = cut_schedule(rows, args.factor)
+    if args.finest_iters is not None:
+        cut[-1] = args.finest_iters
 
     def make(iters=None):
         b = Batch([rows] * n, iters=None if iters is None else [iters] * n)
@@ -410,15 +414,25 @@ def run_learned(args):
     e_ref = {"energy_residual": pe, "residual": inf, "none": inf}[args.fallback]
     r_ref = {"energy_residual": pr, "residual": pr, "none": inf}[args.fallback]
 
+    def misses(e, r, idx):
+        return ~(e <= 1.05 * e_ref[idx]) | ~(r <= 1.05 * r_ref[idx])
+
     def finish():
-        """the rule after the learned solve: optional continuation of the pairs that miss it,
-        then the plain re-solve of those that still do; returns (continued, fell back) flags"""
+        """the rule after the learned solve: optional continuation of the pairs that miss it
+        (re-checked alone), then the plain re-solve of those that still do; returns
+        (continued, fell back) flags"""
+        e, r = learned.metrics()
+        bad = np.nonzero(misses(e, r, slice(None)))[0]
         cont = np.zeros(n, np.uint8)
-        if args.continue_iters > 0:
-            e, r = learned.metrics()
-            cont = (~(e <= 1.05 * e_ref) | ~(r <= 1.05 * r_ref)).astype(np.uint8)
-            learned.continue_finest(np.nonzero(cont)[0], args.continue_iters)
-        return cont, learned.fallback(e_ref, r_ref, 0.05)
+        if args.continue_iters > 0 and len(bad):
+            cont[bad] = 1
+            learned.continue_finest(bad, args.continue_iters)
+            e2, r2 = learned.metrics_pairs(bad)
+            bad = bad[misses(e2, r2, bad)]
+        learned.resolve_plain(bad)
+        flags = np.zeros(n, np.uint8)
+        flags[bad] = 1
+        return cont, flags
 
     def step():
         learned.solve(graph=not args.no_graph)
@@ -496,7 +510,8 @@ def run_learned(args):
         "dtype": "f32 (network bf16 x bf16 -> f32)",
         "data": f"synthetic (seeded RGB-D generator, seeds {args.seed}..{args.seed + n - 1})",
         "config": {"workload": f"batched + learned warm start, {n} pairs of {rows}x{rows * 4 // 3}: "
-                               f"network at the 60x80 level, iterations {cut} (reference / {args.factor}), "
+                               f"network at the 60x80 level, iterations {cut} (reference / {args.factor}"
+                               + (f", finest level {args.finest_iters}" if args.finest_iters is not None else "") + "), "
                                + (f"pairs missing the rule continued by {args.continue_iters} finest-level "
                                   "iterations, " if args.continue_iters else "")
                                + f"fallback rule '{args.fallback}' (5%) with plain re-solves inside the timed "

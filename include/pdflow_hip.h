@@ -237,7 +237,9 @@ int pdf_batch_launch_count(const pdf_batch *b, unsigned *launches);
 int pdf_batch_enable_kernel_timing(pdf_batch *b, int enable);
 int pdf_batch_kernel_timing(pdf_batch *b, uint64_t *launches, double *total_ms, uint64_t *pixel_iters);
 /* planes of pair's last solve on the device: a field of include/pdflow_fields.h (those the
- * batched path keeps, at the finest level) or a pyramid plane (kind, level) */
+ * batched path keeps, at the finest level; the dual carry planes pd_l and p*_l hold the last
+ * coarser level's duals, since nothing reads them after the finest level) or a pyramid plane
+ * (kind, level) */
 int pdf_batch_field_device(pdf_batch *b, unsigned pair, int field, const float **ptr);
 int pdf_batch_pyramid_device(pdf_batch *b, unsigned pair, int kind, unsigned level, const float **ptr,
                              unsigned *rows, unsigned *cols);
@@ -260,6 +262,9 @@ int pdf_batch_set_learned(pdf_batch *b, struct pdf_net *net, const unsigned *pai
  * over pixels with frame-1 depth, u the final flow (bilinear warp of .cu:654-682).
  * Either output may be NULL. */
 int pdf_batch_metrics(pdf_batch *b, float *energy, float *residual);
+/* No counterpart in the reference: pdf_batch_metrics for the listed pairs only; energy[j] and
+ * residual[j] belong to pairs[j] (for re-checking the pairs a continuation touched) */
+int pdf_batch_metrics_pairs(pdf_batch *b, const unsigned *pairs, unsigned count, float *energy, float *residual);
 /* per pair: mean 3-D end-point error (metres) between this batch's flow and ref's over pixels
  * with frame-1 depth (pairs of equal sizes) */
 int pdf_batch_epe(pdf_batch *b, const pdf_batch *ref, float *epe);

@@ -385,7 +385,10 @@ void build_plan(pdf_batch *b, const std::vector<unsigned> &who, bool fallback, P
         add_setup(b, plan, pdf::BatchKernel::RijGradients, 0, active);
         add_setup(b, plan, pdf::BatchKernel::WarpMu, 0, active);
         add_iterations(b, plan, active, [&](const Pair &p) { return iters_of(p, s); });
-        add_setup(b, plan, pdf::BatchKernel::FilterMotion, 0, active);
+        std::vector<unsigned> finest, inner;
+        for (unsigned i : active) (s + 1 == b->pairs[i].ctf_levels ? finest : inner).push_back(i);
+        add_setup(b, plan, pdf::BatchKernel::FilterMotion, 0, inner);
+        add_setup(b, plan, pdf::BatchKernel::FilterMotion, pdf::kFilterFinest, finest);
     }
 }
 
@@ -413,7 +416,7 @@ void build_continue_plan(pdf_batch *b, const std::vector<unsigned> &who, unsigne
             if (b->pairs[i].rows_i * b->pairs[i].cols_i == P) same.push_back(i);
         add_iterations(b, plan, same, [&](const Pair &) { return extra; });
     }
-    add_setup(b, plan, pdf::BatchKernel::FilterMotion, 0, who);
+    add_setup(b, plan, pdf::BatchKernel::FilterMotion, pdf::kFilterFinest, who);
 }
 
 void canonicalize(Pair &p)
@@ -1014,14 +1017,16 @@ int pdf_batch_continue(pdf_batch *b, const unsigned *pairs, unsigned count, unsi
     return run_plan(b, b->fb, tables, false);
 }
 
-int pdf_batch_metrics(pdf_batch *b, float *energy, float *residual)
+namespace {
+
+// per listed pair: the finest level's linearised energy and warped-intensity residual
+int metrics_of(pdf_batch *b, const std::vector<unsigned> &who, float *energy, float *residual)
 {
-    if (!b) return PDF_ERR_ARG;
-    if (!b->planned) return bfail(b, PDF_ERR_STATE, "metrics before any solve");
     Plan m;
     std::vector<pdf::MetricArgs> args;
     std::vector<unsigned> blocks;
-    for (const Pair &p : b->pairs) {
+    for (unsigned i : who) {
+        const Pair &p = b->pairs[i];
         const unsigned L = p.lw;   // the finest level's pyramid image
         pdf::MetricArgs a{};
         a.dct = p.f[PDF_DCT]; a.dcu = p.f[PDF_DCU]; a.dcv = p.f[PDF_DCV];
@@ -1036,6 +1041,7 @@ int pdf_batch_metrics(pdf_batch *b, float *energy, float *residual)
         args.push_back(a);
         blocks.push_back(pdf::learn_blocks(p.rows * p.cols));
     }
+    if (args.empty()) return PDF_OK;
     Launch l{};
     l.kind = Launch::Learn;
     l.learn = pdf::LearnKernel::Metrics;
@@ -1051,17 +1057,38 @@ int pdf_batch_metrics(pdf_batch *b, float *energy, float *residual)
     BHIP(b, hipMemcpyAsync(part.data(), extra, part.size() * sizeof(float), hipMemcpyDeviceToHost, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
     size_t blk = 0;
-    for (size_t i = 0; i < b->pairs.size(); i++) {
+    for (size_t j = 0; j < who.size(); j++) {
         double es = 0, rs = 0, rc_ = 0;
-        for (unsigned q = 0; q < blocks[i]; q++, blk++) {
+        for (unsigned q = 0; q < blocks[j]; q++, blk++) {
             es += part[3 * blk];
             rs += part[3 * blk + 1];
             rc_ += part[3 * blk + 2];
         }
-        if (energy) energy[i] = (float)es;
-        if (residual) residual[i] = (float)(rc_ > 0 ? rs / rc_ : 0.0);
+        if (energy) energy[j] = (float)es;
+        if (residual) residual[j] = (float)(rc_ > 0 ? rs / rc_ : 0.0);
     }
     return PDF_OK;
+}
+
+}  // namespace
+
+int pdf_batch_metrics(pdf_batch *b, float *energy, float *residual)
+{
+    if (!b) return PDF_ERR_ARG;
+    if (!b->planned) return bfail(b, PDF_ERR_STATE, "metrics before any solve");
+    std::vector<unsigned> all(b->pairs.size());
+    for (unsigned i = 0; i < all.size(); i++) all[i] = i;
+    return metrics_of(b, all, energy, residual);
+}
+
+int pdf_batch_metrics_pairs(pdf_batch *b, const unsigned *pairs, unsigned count, float *energy, float *residual)
+{
+    if (!b || (count && !pairs)) return PDF_ERR_ARG;
+    if (!b->planned) return bfail(b, PDF_ERR_STATE, "metrics before any solve");
+    std::vector<unsigned> who(pairs, pairs + count);
+    for (unsigned i : who)
+        if (i >= b->pairs.size()) return bfail(b, PDF_ERR_ARG, "bad pair index");
+    return metrics_of(b, who, energy, residual);
 }
 
 int pdf_batch_epe(pdf_batch *b, const pdf_batch *ref, float *epe)

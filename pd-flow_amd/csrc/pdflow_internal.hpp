@@ -93,6 +93,9 @@ __device__ __forceinline__ BatchItem batch_item(const BatchTable &t, unsigned b)
 __device__ __forceinline__ BatchItem batch_item(const BatchTable &t) { return batch_item(t, blockIdx.x); }
 
 enum class BatchKernel { LevelInit, RijGradients, WarpMu, FilterMotion, Pyramid, WarmInit };
+// FilterMotion launch flag: the listed pairs are at their finest level, whose duals are not
+// carried into the *_l planes (only a next level's upsampling reads them)
+constexpr int kFilterFinest = 1;
 
 // Iterations per launch supported by the temporally blocked kernel.
 constexpr int kTBMaxK = 8;

@@ -255,7 +255,9 @@ __device__ __forceinline__ Sat saturated(const LevelArgs &a, unsigned i)
     return {du + a.f[PDF_DU_PREV][i], dv + a.f[PDF_DV_PREV][i], dw};
 }
 
-__device__ __forceinline__ void filter_motion_body(const LevelArgs &a, unsigned tile)
+// carry_duals: copy pd and the six duals into the *_l carry planes that the next level's
+// upsampling reads (.cu:1134-1147); a pair's finest level has no next level
+__device__ __forceinline__ void filter_motion_body(const LevelArgs &a, unsigned tile, bool carry_duals = true)
 {
     // the tile plus a one-pixel halo: saturated solution, frame-1 depth and ddt of each pixel,
     // computed once per pixel and read by the (up to) 9 pixels whose window contains it
@@ -332,13 +334,15 @@ __device__ __forceinline__ void filter_motion_body(const LevelArgs &a, unsigned 
     a.f[PDF_DU_L][index] = ou;
     a.f[PDF_DV_L][index] = ov;
     a.f[PDF_DW_L][index] = ow;
-    a.f[PDF_PD_L][index] = a.f[PDF_PD][index];
-    a.f[PDF_PUU_L][index] = a.f[PDF_PUU][index];
-    a.f[PDF_PUV_L][index] = a.f[PDF_PUV][index];
-    a.f[PDF_PVU_L][index] = a.f[PDF_PVU][index];
-    a.f[PDF_PVV_L][index] = a.f[PDF_PVV][index];
-    a.f[PDF_PWU_L][index] = a.f[PDF_PWU][index];
-    a.f[PDF_PWV_L][index] = a.f[PDF_PWV][index];
+    if (carry_duals) {
+        a.f[PDF_PD_L][index] = a.f[PDF_PD][index];
+        a.f[PDF_PUU_L][index] = a.f[PDF_PUU][index];
+        a.f[PDF_PUV_L][index] = a.f[PDF_PUV][index];
+        a.f[PDF_PVU_L][index] = a.f[PDF_PVU][index];
+        a.f[PDF_PVV_L][index] = a.f[PDF_PVV][index];
+        a.f[PDF_PWU_L][index] = a.f[PDF_PWU][index];
+        a.f[PDF_PWV_L][index] = a.f[PDF_PWV][index];
+    }
     // K15 (.cu:1150-1167)
     if (d0 > 0.f) {
         a.f[PDF_DX][index] = ow;
@@ -427,10 +431,11 @@ __global__ void __launch_bounds__(kBlock) kb_warp_mu(BatchTable t)
     const BatchItem it = batch_item(t);
     warp_mu_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, 0);
 }
-__global__ void __launch_bounds__(kBlock) kb_filter_motion(BatchTable t)
+// flag kFilterFinest: the pairs' finest level (no dual carry)
+__global__ void __launch_bounds__(kBlock) kb_filter_motion(BatchTable t, int flag)
 {
     const BatchItem it = batch_item(t);
-    filter_motion_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block);
+    filter_motion_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, (flag & kFilterFinest) == 0);
 }
 __global__ void __launch_bounds__(kBlock) kb_pyramid(BatchTable t)
 {
@@ -499,7 +504,7 @@ hipError_t launch_batch(BatchKernel k, const BatchTable &t, unsigned total_block
     case BatchKernel::LevelInit: hipLaunchKernelGGL(kb_level_init, grid, block, 0, s, t, flag); break;
     case BatchKernel::RijGradients: hipLaunchKernelGGL(kb_rij_gradients, grid, block, 0, s, t); break;
     case BatchKernel::WarpMu: hipLaunchKernelGGL(kb_warp_mu, grid, block, 0, s, t); break;
-    case BatchKernel::FilterMotion: hipLaunchKernelGGL(kb_filter_motion, grid, block, 0, s, t); break;
+    case BatchKernel::FilterMotion: hipLaunchKernelGGL(kb_filter_motion, grid, block, 0, s, t, flag); break;
     case BatchKernel::Pyramid: hipLaunchKernelGGL(kb_pyramid, grid, block, 0, s, t); break;
     case BatchKernel::WarmInit: hipLaunchKernelGGL(kb_warm_init, grid, block, 0, s, t); break;
     default: return hipErrorInvalidValue;

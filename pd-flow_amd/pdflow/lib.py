@@ -121,6 +121,7 @@ def hip_library():
         "pdf_batch_enable_kernel_timing": ([vp, c_int], c_int),
         "pdf_device_copy": ([vp, vp, ctypes.c_size_t], c_int),
         "pdf_batch_continue": ([vp, ctypes.POINTER(c_uint), c_uint, c_uint], c_int),
+        "pdf_batch_metrics_pairs": ([vp, ctypes.POINTER(c_uint), c_uint, fp, fp], c_int),
         "pdf_check_exact_sequence": ([c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_uint32)], c_int),
         "pdf_set_tb_shape": ([vp, c_int], c_int),
@@ -522,6 +523,17 @@ class Batch:
         n = len(pairs)
         arr = (ctypes.c_uint * max(n, 1))(*pairs)
         self._chk(self.lib.pdf_batch_resolve_plain(self.ctx, arr, n))
+
+    def metrics_pairs(self, pairs):
+        """(energy, residual) of the listed pairs only (pdf_batch_metrics_pairs)"""
+        pairs = [int(i) for i in pairs]
+        n = len(pairs)
+        e = np.zeros(n, np.float32)
+        r = np.zeros(n, np.float32)
+        if n:
+            arr = (ctypes.c_uint * n)(*pairs)
+            self._chk(self.lib.pdf_batch_metrics_pairs(self.ctx, arr, n, _fptr(e), _fptr(r)))
+        return e, r
 
     def continue_finest(self, pairs, extra_iters):
         """extra_iters more iterations of the listed pairs at their finest level, continuing from

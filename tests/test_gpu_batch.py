@@ -78,6 +78,10 @@ def test_continue_equals_a_longer_finest_level(gpu_available):
     eb, rb = b.metrics()
     np.testing.assert_array_equal(ea, eb)
     np.testing.assert_array_equal(ra, rb)
+    # the metrics of a subset, in the listed order, are the full batch's for those pairs
+    es, rs = a.metrics_pairs([3, 0])
+    np.testing.assert_array_equal(es, ea[[3, 0]])
+    np.testing.assert_array_equal(rs, ra[[3, 0]])
     # a new solve of batch a replays its own schedule from scratch
     a.solve(graph=True)
     a.synchronize()
