@@ -2,8 +2,8 @@
 
 Default workload (--workload learned): 256 VGA (480x640) RGB-D pairs per step, batched with the
 learned warm start -- both pyramids of every pair, one network pass over their 60x80 levels,
-the solves from there with every level's iterations divided by --factor (2), 25 more finest-level
-iterations (--continue-iters) for the pairs that miss the rule, then the fallback rule (final
+the solves from there with every level's iterations divided by --factor (2), up to two stages of 16
+more finest-level iterations (--continue-iters) for the pairs that miss the rule, then the rule (final
 energy or warped residual more than 5% above the plain solver's) and the plain re-solve of the
 pairs it rejects, all inside the timed region.  The line also carries the plain single-pair and
 plain batched rates of the same pairs, end-point errors against the plain solve and against the
@@ -110,10 +110,11 @@ def main():
                          "of HBM byte counters under rocprofv3 --pmc)")
     ap.add_argument("--finest-iters", type=int, default=None, metavar="N",
                     help="learned: the finest level's iteration count in place of the cut schedule's")
-    ap.add_argument("--continue-iters", type=int, default=25, metavar="N",
+    ap.add_argument("--continue-iters", default="16,16", metavar="N[,N...]",
                     help="learned: pairs that miss the fallback rule first get N more finest-level "
-                         "iterations from where they stand (pdf_batch_continue); the rule then "
-                         "re-solves those that still miss it")
+                         "iterations from where they stand (pdf_batch_continue), stage by stage for "
+                         "a list, each stage re-checking the rule; the plain path then re-solves "
+                         "those that still miss it (0: no continuation)")
     ap.add_argument("--fallback", choices=("energy_residual", "residual", "none"), default="energy_residual",
                     help="learned: rule deciding which pairs the plain path re-solves (tolerance 5%%)")
     args = ap.parse_args()
@@ -388,6 +389,7 @@ def run_learned(args):
     *planes, gt = synth_pairs_float(args.seed, n, 480 // cam, 640 // cam, with_gt=True)
     net = Net(max_batch=n, h=60, w=80)
     net.load(args.weights)
+    stages = [int(x) for x in str(args.continue_iters).split(",") if int(x) > 0]
     cut = cut_schedule(rows, args.factor)
     if args.finest_iters is not None:
         cut[-1] = args.finest_iters
@@ -424,9 +426,11 @@ def run_learned(args):
         e, r = learned.metrics()
         bad = np.nonzero(misses(e, r, slice(None)))[0]
         cont = np.zeros(n, np.uint8)
-        if args.continue_iters > 0 and len(bad):
+        for extra in stages:
+            if not len(bad):
+                break
             cont[bad] = 1
-            learned.continue_finest(bad, args.continue_iters)
+            learned.continue_finest(bad, extra)
             e2, r2 = learned.metrics_pairs(bad)
             bad = bad[misses(e2, r2, bad)]
         learned.resolve_plain(bad)
@@ -512,8 +516,8 @@ def run_learned(args):
         "config": {"workload": f"batched + learned warm start, {n} pairs of {rows}x{rows * 4 // 3}: "
                                f"network at the 60x80 level, iterations {cut} (reference / {args.factor}"
                                + (f", finest level {args.finest_iters}" if args.finest_iters is not None else "") + "), "
-                               + (f"pairs missing the rule continued by {args.continue_iters} finest-level "
-                                  "iterations, " if args.continue_iters else "")
+                               + (f"pairs missing the rule continued by {'+'.join(map(str, stages))} finest-level "
+                                  "iterations, " if stages else "")
                                + f"fallback rule '{args.fallback}' (5%) with plain re-solves inside the timed "
                                "region" + (", hipGraph" if not args.no_graph else ""),
                    "pairs_per_step": n, "weights": os.path.relpath(args.weights, ROOT)},
@@ -528,7 +532,7 @@ def run_learned(args):
                       "epe_vs_truth_ablation": stats(gt_abl),
                       "energy_ratio_learned": stats(le / pe), "residual_ratio_learned": stats(lr_ / pr),
                       "continued_fraction": round(float(cont.mean()), 4),
-                      "continue_iters": args.continue_iters,
+                      "continue_iters": stages,
                       "fallback_fraction": round(float(flags.mean()), 4)},
         "roofline": {"bound": "hbm", "kernel": "kb_pd_tb<K, 32, 32> (batched, K = 2 to 5 primal-dual iterations per launch)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
