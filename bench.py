@@ -2,8 +2,8 @@
 
 Default workload (--workload learned): 256 VGA (480x640) RGB-D pairs per step, batched with the
 learned warm start -- both pyramids of every pair, one network pass over their 60x80 levels,
-the solves from there with every level's iterations divided by --factor (2), up to two stages of 16
-more finest-level iterations (--continue-iters) for the pairs that miss the rule, then the rule (final
+the solves from there with every level's iterations divided by --factor (2.5), up to three stages of
+16 more finest-level iterations (--continue-iters) for the pairs that miss the rule, then the rule (final
 energy or warped residual more than 5% above the plain solver's) and the plain re-solve of the
 pairs it rejects, all inside the timed region.  The line also carries the plain single-pair and
 plain batched rates of the same pairs, end-point errors against the plain solve and against the
@@ -33,7 +33,7 @@ BYTES_PER_PIXEL_ITER = 148     # 11 constant planes read + 13 state planes read 
 # of this workload, scaled by the counters' readings for a plain copy of known size (fetch x 2.0,
 # write x 1.0); the passes and the per-level figures are in
 # profiles/pd_iteration_batched_learned_vga256_hbm_traffic.txt
-TRAFFIC_TB_LEARNED_VGA256 = {2.0: 4.789e9, 4.0: 4.885e9}
+TRAFFIC_TB_LEARNED_VGA256 = {2.0: 4.789e9, 2.5: 4.658e9, 4.0: 4.885e9}
 
 
 def log(msg):
@@ -98,9 +98,10 @@ def main():
                          "learned: --batch-pairs pairs of --rows per step, network warm start at "
                          "60x80, iterations / --factor, fallback rule")
     ap.add_argument("--batch-pairs", type=int, default=256)
-    ap.add_argument("--factor", type=float, default=2,
-                    help="learned: every level's iteration count divided by this (2 gives the highest "
-                         "throughput once the fallback re-solves are counted)")
+    ap.add_argument("--factor", type=float, default=2.5,
+                    help="learned: every level's iteration count divided by this (2.5 with three "
+                         "continuation stages gives the highest throughput once continuation and "
+                         "fallback re-solves are counted)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "pd-flow_amd", "pdflow", "weights", "warmstart_qvga.bin"))
     ap.add_argument("--autotune", default=None, metavar="CACHE",
                     help="batched/learned: pick the kernel configuration with pdf_batch_autotune "
@@ -110,7 +111,7 @@ def main():
                          "of HBM byte counters under rocprofv3 --pmc)")
     ap.add_argument("--finest-iters", type=int, default=None, metavar="N",
                     help="learned: the finest level's iteration count in place of the cut schedule's")
-    ap.add_argument("--continue-iters", default="16,16", metavar="N[,N...]",
+    ap.add_argument("--continue-iters", default="16,16,16", metavar="N[,N...]",
                     help="learned: pairs that miss the fallback rule first get N more finest-level "
                          "iterations from where they stand (pdf_batch_continue), stage by stage for "
                          "a list, each stage re-checking the rule; the plain path then re-solves "
