@@ -109,6 +109,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run a plain device copy of BYTES three times (the calibration point "
                          "of HBM byte counters under rocprofv3 --pmc)")
+    ap.add_argument("--schedule", default=None, metavar="N,N,...",
+                    help="learned: the whole per-level iteration schedule (coarse to fine) in place of "
+                         "the cut one")
     ap.add_argument("--finest-iters", type=int, default=None, metavar="N",
                     help="learned: the finest level's iteration count in place of the cut schedule's")
     ap.add_argument("--continue-iters", default="16,16,16", metavar="N[,N...]",
@@ -392,6 +395,10 @@ def run_learned(args):
     net.load(args.weights)
     stages = [int(x) for x in str(args.continue_iters).split(",") if int(x) > 0]
     cut = cut_schedule(rows, args.factor)
+    if args.schedule:
+        cut = [int(x) for x in args.schedule.split(",")]
+        if len(cut) != len(cut_schedule(rows, args.factor)):
+            raise SystemExit(f"bench.py: --schedule needs {len(cut_schedule(rows, args.factor))} levels")
     if args.finest_iters is not None:
         cut[-1] = args.finest_iters
 
@@ -515,8 +522,9 @@ def run_learned(args):
         "dtype": "f32 (network bf16 x bf16 -> f32)",
         "data": f"synthetic (seeded RGB-D generator, seeds {args.seed}..{args.seed + n - 1})",
         "config": {"workload": f"batched + learned warm start, {n} pairs of {rows}x{rows * 4 // 3}: "
-                               f"network at the 60x80 level, iterations {cut} (reference / {args.factor}"
-                               + (f", finest level {args.finest_iters}" if args.finest_iters is not None else "") + "), "
+                               f"network at the 60x80 level, iterations {cut} "
+                               + ("(given), " if args.schedule else f"(reference / {args.factor}"
+                                  + (f", finest level {args.finest_iters}" if args.finest_iters is not None else "") + "), ")
                                + (f"pairs missing the rule continued by {'+'.join(map(str, stages))} finest-level "
                                   "iterations, " if stages else "")
                                + f"fallback rule '{args.fallback}' (5%) with plain re-solves inside the timed "
