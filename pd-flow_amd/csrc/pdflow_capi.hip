@@ -830,7 +830,7 @@ int pdf_device_copy(void *dst, const void *src, size_t bytes)
 
 int pdf_check_exact_sequence(int which, uint32_t lo, uint32_t hi, uint64_t *mismatches, uint32_t *first_mismatch)
 {
-    if (which < 0 || which > 2 || lo > hi || !mismatches || !first_mismatch) return PDF_ERR_ARG;
+    if (which < 0 || which > 4 || lo > hi || !mismatches || !first_mismatch) return PDF_ERR_ARG;
     void *d = nullptr;
     if (hipMalloc(&d, 16) != hipSuccess) return PDF_ERR_HIP;
     const uint32_t init[4] = {0u, 0u, 0xffffffffu, 0u};

@@ -122,7 +122,7 @@ __device__ __forceinline__ void pd_fused_body(const PDArgs &a, unsigned bx, unsi
             g2w = rj * (sW[r + 1] - w_c);
         }
         const float sx = sigma_p_of(li, ri), sy = sigma_p_of(li, rj);
-        const float swx = 0.5f / (ri * ld + 1e-10f), swy = 0.5f / (rj * ld + 1e-10f);
+        const float swx = div_half_or_one(0.5f, ri * ld + 1e-10f), swy = div_half_or_one(0.5f, rj * ld + 1e-10f);
         TVDual p;
         p.puu = pin[0] + sx * li * g1u;
         p.puv = pin[1] + sy * li * g2u;
@@ -255,8 +255,8 @@ __device__ __forceinline__ void derive_px(Px &q, float riu, float rjv, float li,
     // the first products of the reference expressions (.cu:822-834) are per-launch constants
     q.sxl = sigma_p_of(li, q.ri) * li;
     q.syl = sigma_p_of(li, q.rj) * li;
-    q.swxl = (0.5f / (q.ri * ld + 1e-10f)) * ld;
-    q.swyl = (0.5f / (q.rj * ld + 1e-10f)) * ld;
+    q.swxl = div_half_or_one(0.5f, q.ri * ld + 1e-10f) * ld;
+    q.swyl = div_half_or_one(0.5f, q.rj * ld + 1e-10f) * ld;
     q.spdm = sigma_pd_of(q.mu, q.ddu, q.ddv) * q.mu;
     float acu_r = q.ri + q.rj;                                   // .cu:801-803
     if (q.gu > 0) acu_r += riu;

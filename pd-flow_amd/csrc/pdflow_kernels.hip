@@ -554,9 +554,15 @@ __global__ void __launch_bounds__(256) k_check_exact(int which, unsigned lo, uns
         } else if (which == 1) {
             fast = rcp_rn_ge1(x);
             ieee = 1.f / x;
-        } else {
+        } else if (which == 2) {
             fast = inv_sqrt_rn_gt1(x);
             ieee = 1.f / sqrtf(x);
+        } else if (which == 3) {
+            fast = div_rn_half_or_one(1.f, x);
+            ieee = 1.f / x;
+        } else {
+            fast = div_rn_half_or_one(0.5f, x);
+            ieee = 0.5f / x;
         }
         if (__float_as_uint(fast) != __float_as_uint(ieee)) {
             bad++;

@@ -9,23 +9,6 @@ namespace pdf {
 
 __device__ __forceinline__ float rhyp(float a, float b) { return 1.f / sqrtf(a * a + b * b); }
 
-__device__ __forceinline__ float sigma_pd_of(float mu_uv, float ddu, float ddv)
-{
-    return 1.f / (mu_uv * (1.f + fabsf(ddu) + fabsf(ddv)) + 1e-10f);   // .cu:791
-}
-__device__ __forceinline__ float sigma_p_of(float lam, float r)
-{
-    return 0.5f / (lam * r + 1e-10f);                                    // .cu:792-795
-}
-__device__ __forceinline__ float tau_uv_of(float mu_uv, float dd, float lam, float acu_r)
-{
-    return 1.f / (mu_uv * fabsf(dd) + lam * acu_r + 1e-10f);             // .cu:805-806
-}
-__device__ __forceinline__ float tau_w_of(float mu_uv, float lam, float acu_r)
-{
-    return 1.f / (mu_uv + lam * acu_r + 1e-10f);                         // .cu:807
-}
-
 // ------------------------------------------------------------------ exact short sequences
 // IEEE FP32 sqrt and division on gfx950 are v_sqrt_f32 / v_rcp_f32 estimates made exact by FMA
 // steps, wrapped in range handling: a 2^32 pre-scale and class test for sqrt, v_div_scale /
@@ -57,10 +40,46 @@ __device__ __forceinline__ float rcp_rn_ge1(float d)
     return __builtin_fmaf(__builtin_fmaf(-d, q, 1.f), r, q);
 }
 
+// n / d for n = 0.5 or 1 and 2^-94 <= d <= 2^124: the same steps with the numerator
+__device__ __forceinline__ float div_rn_half_or_one(float n, float d)
+{
+    const float r0 = __builtin_amdgcn_rcpf(d);
+    const float r = __builtin_fmaf(__builtin_fmaf(-d, r0, 1.f), r0, r0);
+    const float q = n * r;
+    const float q1 = __builtin_fmaf(__builtin_fmaf(-d, q, n), r, q);
+    return __builtin_fmaf(__builtin_fmaf(-d, q1, n), r, q1);
+}
+
+// n / d for n = 0.5 or 1 and any d: the short sequence in its range, the full division (a branch
+// no lane takes for the step sizes' positive, guarded denominators) elsewhere
+__device__ __forceinline__ float div_half_or_one(float n, float d)
+{
+    if (__builtin_expect(d >= 0x1p-94f && d <= 0x1p124f, 1)) return div_rn_half_or_one(n, d);
+    return n / d;
+}
+
 // 1.f / sqrtf(x) for x > 1 (1/+inf = 0)
 __device__ __forceinline__ float inv_sqrt_rn_gt1(float x)
 {
     return x < __builtin_inff() ? rcp_rn_ge1(sqrt_rn_gt1(x)) : 0.f;
+}
+
+
+__device__ __forceinline__ float sigma_pd_of(float mu_uv, float ddu, float ddv)
+{
+    return div_half_or_one(1.f, mu_uv * (1.f + fabsf(ddu) + fabsf(ddv)) + 1e-10f);   // .cu:791
+}
+__device__ __forceinline__ float sigma_p_of(float lam, float r)
+{
+    return div_half_or_one(0.5f, lam * r + 1e-10f);                      // .cu:792-795
+}
+__device__ __forceinline__ float tau_uv_of(float mu_uv, float dd, float lam, float acu_r)
+{
+    return div_half_or_one(1.f, mu_uv * fabsf(dd) + lam * acu_r + 1e-10f);   // .cu:805-806
+}
+__device__ __forceinline__ float tau_w_of(float mu_uv, float lam, float acu_r)
+{
+    return div_half_or_one(1.f, mu_uv + lam * acu_r + 1e-10f);           // .cu:807
 }
 
 __device__ __forceinline__ void project(float &a, float &b)

@@ -156,7 +156,7 @@ def device_count():
     return n.value
 
 
-EXACT_SQRT, EXACT_RCP, EXACT_INV_SQRT = 0, 1, 2
+EXACT_SQRT, EXACT_RCP, EXACT_INV_SQRT, EXACT_ONE_OVER, EXACT_HALF_OVER = 0, 1, 2, 3, 4
 
 
 def check_exact_sequence(which, lo, hi):

@@ -1,4 +1,5 @@
-"""Exhaustive GPU checks of the short exact sequences in pd-flow_amd/csrc/pdflow_pixel.hpp.
+"""Exhaustive GPU checks of the short exact sequences in pd-flow_amd/csrc/pdflow_pixel.hpp (the disc
+projection's square root and reciprocal, and the step sizes' divisions of 0.5 and 1).
 
 The iteration kernels' unit-disc projection (pdflow_cudalib.cu:849-859) takes 1/sqrt(x) on
 x > 1. It uses a square root and a reciprocal without the range handling that the compiler
@@ -14,6 +15,8 @@ ONE = 0x3F800000            # 1.0f
 NEXT_AFTER_ONE = 0x3F800001
 INF = 0x7F800000
 TWO_TO_64 = 0x5F800000      # 2^64
+TWO_TO_M94 = 0x10800000     # 2^-94
+TWO_TO_124 = 0x7D800000     # 2^124
 
 
 @pytest.fixture(scope="module")
@@ -26,6 +29,8 @@ def lib(gpu_available):
     (0, NEXT_AFTER_ONE, INF, "sqrt on (1, inf]"),
     (1, ONE, TWO_TO_64 - 1, "reciprocal on [1, 2^64)"),
     (2, NEXT_AFTER_ONE, INF, "1/sqrt on (1, inf]"),
+    (3, TWO_TO_M94, TWO_TO_124, "1/d on [2^-94, 2^124]"),
+    (4, TWO_TO_M94, TWO_TO_124, "0.5/d on [2^-94, 2^124]"),
 ])
 def test_sequence_matches_ieee_on_its_whole_range(lib, which, lo, hi, what):
     n, first = lib.check_exact_sequence(which, lo, hi)
@@ -34,6 +39,6 @@ def test_sequence_matches_ieee_on_its_whole_range(lib, which, lo, hi, what):
 
 def test_bad_arguments_are_refused(lib):
     with pytest.raises(lib.PDFlowError):
-        lib.check_exact_sequence(3, ONE, ONE)
+        lib.check_exact_sequence(5, ONE, ONE)
     with pytest.raises(lib.PDFlowError):
         lib.check_exact_sequence(0, INF, ONE)
