@@ -85,6 +85,21 @@ def test_create_rejects_bad_arguments_before_touching_the_device():
         assert getattr(lib, "pdf_" + st)(None) == 1
 
 
+def test_batch_entry_points_reject_null_handles_before_touching_the_device():
+    lib = plib.hip_library()
+    pairs = (ctypes.c_uint * 2)(0, 1)
+    e = (ctypes.c_float * 2)()
+    r = (ctypes.c_float * 2)()
+    assert lib.pdf_batch_continue(None, pairs, 2, 16) == 1
+    assert lib.pdf_batch_metrics_pairs(None, pairs, 2, e, r) == 1
+    assert lib.pdf_batch_resolve_plain(None, pairs, 2) == 1
+    assert lib.pdf_batch_metrics(None, e, r) == 1
+    n, first = ctypes.c_uint64(), ctypes.c_uint32()
+    assert lib.pdf_check_exact_sequence(5, 0, 1, ctypes.byref(n), ctypes.byref(first)) == 1
+    assert lib.pdf_check_exact_sequence(0, 2, 1, ctypes.byref(n), ctypes.byref(first)) == 1
+    assert lib.pdf_check_exact_sequence(0, 0, 1, None, ctypes.byref(first)) == 1
+
+
 def test_field_table_matches_header():
     text = open(os.path.join(ROOT, "include", "pdflow_fields.h")).read()
     body = text[text.index("enum pdf_field_id"):text.index("PDF_NUM_FIELDS")]
