@@ -5,7 +5,8 @@ learned warm start -- both pyramids of every pair, one network pass over their 6
 the solves from there with every level's iterations divided by --factor (2.5), up to three stages of
 16 more finest-level iterations (--continue-iters) for the pairs that miss the rule, then the rule (final
 energy or warped residual more than 5% above the plain solver's) and the plain re-solve of the
-pairs it rejects, all inside the timed region.  The line also carries the plain single-pair and
+pairs it rejects, all inside the timed region; two batches are in flight, so that one batch's
+continuation and fallback overlap the next batch's solve (--pipeline).  The line also carries the plain single-pair and
 plain batched rates of the same pairs, end-point errors against the plain solve and against the
 rendered ground truth, and the fallback fraction.
 
@@ -119,6 +120,10 @@ def main():
                          "iterations from where they stand (pdf_batch_continue), stage by stage for "
                          "a list, each stage re-checking the rule; the plain path then re-solves "
                          "those that still miss it (0: no continuation)")
+    ap.add_argument("--pipeline", type=int, choices=(0, 1), default=1,
+                    help="learned: 1 = two batches in flight on their own streams, so that one batch's "
+                         "continuation and fallback overlap the next batch's solve; 0 = one batch, "
+                         "steps back to back")
     ap.add_argument("--fallback", choices=("energy_residual", "residual", "none"), default="energy_residual",
                     help="learned: rule deciding which pairs the plain path re-solves (tolerance 5%%)")
     args = ap.parse_args()
@@ -419,6 +424,17 @@ def run_learned(args):
     learned.set_learned(net)
     if args.autotune:
         log(f"bench: autotuned (shape, K, ms) = {learned.autotune(args.autotune)}")
+    # a second batch of the same pairs with its own network instance (activations are per
+    # network) and stream, for the pipelined steps
+    batches = [learned]
+    if args.pipeline:
+        net2 = Net(max_batch=n, h=60, w=80)
+        net2.load(args.weights)
+        learned2 = make(cut)
+        learned2.set_learned(net2)
+        if args.autotune:
+            learned2.autotune(args.autotune)
+        batches.append(learned2)
     ablation = make(cut)
     inf = np.full(n, np.inf, np.float32)
     e_ref = {"energy_residual": pe, "residual": inf, "none": inf}[args.fallback]
@@ -427,36 +443,47 @@ def run_learned(args):
     def misses(e, r, idx):
         return ~(e <= 1.05 * e_ref[idx]) | ~(r <= 1.05 * r_ref[idx])
 
-    def finish():
-        """the rule after the learned solve: optional continuation of the pairs that miss it
-        (re-checked alone), then the plain re-solve of those that still do; returns
+    def finish(b):
+        """the rule after the learned solve of batch b: optional continuation of the pairs that
+        miss it (re-checked alone), then the plain re-solve of those that still do; returns
         (continued, fell back) flags"""
-        e, r = learned.metrics()
+        e, r = b.metrics()
         bad = np.nonzero(misses(e, r, slice(None)))[0]
         cont = np.zeros(n, np.uint8)
         for extra in stages:
             if not len(bad):
                 break
             cont[bad] = 1
-            learned.continue_finest(bad, extra)
-            e2, r2 = learned.metrics_pairs(bad)
+            b.continue_finest(bad, extra)
+            e2, r2 = b.metrics_pairs(bad)
             bad = bad[misses(e2, r2, bad)]
-        learned.resolve_plain(bad)
+        b.resolve_plain(bad)
         flags = np.zeros(n, np.uint8)
         flags[bad] = 1
         return cont, flags
 
-    def step():
-        learned.solve(graph=not args.no_graph)
-        return finish()[1]
+    def run_steps(k):
+        """k steps, each a whole batch (solve, continuation, fallback).  Pipelined: the next
+        batch's solve is launched on its own stream before this batch's finish, whose host
+        synchronisations wait on this batch's stream only."""
+        graph = not args.no_graph
+        nb = len(batches)
+        batches[0].solve(graph=graph)
+        flags = None
+        for s in range(k):
+            cur = batches[s % nb]
+            if nb > 1 and s + 1 < k:
+                batches[(s + 1) % nb].solve(graph=graph)
+            flags = finish(cur)[1]
+            if nb == 1 and s + 1 < k:
+                cur.solve(graph=graph)
+        for b in batches:
+            b.synchronize()
+        return flags
 
-    for _ in range(max(1, args.warmup)):
-        flags = step()
-    learned.synchronize()
+    flags = run_steps(max(1, args.warmup))
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        flags = step()
-    learned.synchronize()
+    flags = run_steps(args.steps)
     el = time.perf_counter() - t0
     pairs_s = n * args.steps / el
     log(f"bench: {args.steps} learned steps in {el:.3f} s -> {pairs_s:.1f} pairs/s "
@@ -489,7 +516,7 @@ def run_learned(args):
     epe_l = learned.epe(plain)
     le, lr_ = learned.metrics()
     gt_learned = gt_epe(learned)
-    cont, flags = finish()
+    cont, flags = finish(learned)
     epe_f = learned.epe(plain)
     ablation.solve(graph=True)
     ablation.synchronize()
@@ -528,8 +555,11 @@ def run_learned(args):
                                + (f"pairs missing the rule continued by {'+'.join(map(str, stages))} finest-level "
                                   "iterations, " if stages else "")
                                + f"fallback rule '{args.fallback}' (5%) with plain re-solves inside the timed "
-                               "region" + (", hipGraph" if not args.no_graph else ""),
-                   "pairs_per_step": n, "weights": os.path.relpath(args.weights, ROOT)},
+                               "region" + (", hipGraph" if not args.no_graph else "")
+                               + (", two batches in flight (one's continuation and fallback overlap the "
+                                  "next one's solve)" if args.pipeline else ""),
+                   "pairs_per_step": n, "batches_in_flight": len(batches),
+                   "weights": os.path.relpath(args.weights, ROOT)},
         "vs_plain": {"plain_single_pair_pairs_s": round(plain_single, 3),
                      "plain_batched_pairs_s": round(plain_batched, 3),
                      "learned_without_fallback_pairs_s": round(learned_only, 3),
