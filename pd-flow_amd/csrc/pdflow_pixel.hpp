@@ -13,9 +13,11 @@ __device__ __forceinline__ float rhyp(float a, float b) { return 1.f / sqrtf(a *
 // IEEE FP32 sqrt and division on gfx950 are v_sqrt_f32 / v_rcp_f32 estimates made exact by FMA
 // steps, wrapped in range handling: a 2^32 pre-scale and class test for sqrt, v_div_scale /
 // v_div_fmas / v_div_fixup for division.  On the ranges below the range handling is the
-// identity (no operand is tiny, huge, zero, denormal or non-finite), so these functions are the
-// same FMA steps without it and return the same bits as sqrtf and "/".
-// tests/test_gpu_exact.py sweeps every float of each range against sqrtf and "/".
+// identity (no operand is tiny, huge, zero, denormal or non-finite).  The square root keeps the
+// compiler's FMA steps; for reciprocals of these ranges the v_rcp_f32 estimate after one Newton
+// step is already the correctly rounded quotient, so the residual corrections are dropped.
+// tests/test_gpu_exact.py sweeps every float of each range against sqrtf and "/" and finds the
+// same bits everywhere.
 
 // sqrtf(x) for 1 < x <= FLT_MAX (and +inf): the estimate s, then the neighbour of s whose
 // residual x - s'*s shows that s was one step off (one below, one above).
@@ -30,25 +32,20 @@ __device__ __forceinline__ float sqrt_rn_gt1(float x)
     return r_above > 0.f ? above : t;
 }
 
-// 1.f / d for 1 <= d < 2^64: the v_rcp_f32 estimate refined by one Newton step, then two
-// residual corrections of the quotient.
-__device__ __forceinline__ float rcp_rn_ge1(float d)
+// 1 / d, correctly rounded for 2^-94 <= d <= 2^124: the v_rcp_f32 estimate refined by one
+// Newton step
+__device__ __forceinline__ float rcp_rn(float d)
 {
     const float r0 = __builtin_amdgcn_rcpf(d);
-    const float r = __builtin_fmaf(__builtin_fmaf(-d, r0, 1.f), r0, r0);
-    const float q = __builtin_fmaf(__builtin_fmaf(-d, r, 1.f), r, r);
-    return __builtin_fmaf(__builtin_fmaf(-d, q, 1.f), r, q);
+    return __builtin_fmaf(__builtin_fmaf(-d, r0, 1.f), r0, r0);
 }
 
-// n / d for n = 0.5 or 1 and 2^-94 <= d <= 2^124: the same steps with the numerator
-__device__ __forceinline__ float div_rn_half_or_one(float n, float d)
-{
-    const float r0 = __builtin_amdgcn_rcpf(d);
-    const float r = __builtin_fmaf(__builtin_fmaf(-d, r0, 1.f), r0, r0);
-    const float q = n * r;
-    const float q1 = __builtin_fmaf(__builtin_fmaf(-d, q, n), r, q);
-    return __builtin_fmaf(__builtin_fmaf(-d, q1, n), r, q1);
-}
+// 1.f / d for 1 <= d < 2^64
+__device__ __forceinline__ float rcp_rn_ge1(float d) { return rcp_rn(d); }
+
+// n / d for n = 0.5 or 1 and 2^-94 <= d <= 2^124: n times the correctly rounded 1/d (scaling by
+// 0.5 is exact in this range)
+__device__ __forceinline__ float div_rn_half_or_one(float n, float d) { return n * rcp_rn(d); }
 
 // n / d for n = 0.5 or 1 and any d: the short sequence in its range, the full division (a branch
 // no lane takes for the step sizes' positive, guarded denominators) elsewhere

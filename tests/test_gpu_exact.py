@@ -2,10 +2,11 @@
 projection's square root and reciprocal, and the step sizes' divisions of 0.5 and 1).
 
 The iteration kernels' unit-disc projection (pdflow_cudalib.cu:849-859) takes 1/sqrt(x) on
-x > 1. It uses a square root and a reciprocal without the range handling that the compiler
-wraps around v_sqrt_f32 and v_rcp_f32. On those ranges the range handling does nothing, so every
-result must equal sqrtf and "/" bit for bit. These tests evaluate every float of each range
-(about 10^9 inputs apiece) on the GPU and count differing bits."""
+x > 1. It uses a square root without the range handling that the compiler wraps around
+v_sqrt_f32, and a reciprocal that is the v_rcp_f32 estimate after one Newton step, without the
+compiler's residual corrections and range handling. Every result must equal sqrtf and "/" bit for
+bit. These tests evaluate every float of each range (about 10^9 inputs apiece) on the GPU and
+count differing bits."""
 
 import pytest
 
