@@ -300,8 +300,8 @@ int pdf_device_copy(void *dst, const void *src, size_t bytes);
 
 /* No counterpart in the reference: checks the iteration kernels' short exact arithmetic
  * sequences against the compiler's IEEE operations on every float whose bit pattern lies in
- * [lo, hi] (blocking).  which = 0: square root (valid for 1 < x <= +inf), 1: reciprocal
- * (1 <= x < 2^64), 2: reciprocal square root (x > 1), 3: 1/x and 4: 0.5/x (2^-94 <= x <= 2^124).  *mismatches = inputs whose results
+ * [lo, hi] (blocking).  which = 0: square root (valid for 1 <= x <= +inf), 1: reciprocal
+ * (1 <= x < 2^64), 2: reciprocal square root (x >= 1), 3: 1/x and 4: 0.5/x (2^-94 <= x <= 2^124).  *mismatches = inputs whose results
  * differ in any bit, *first_mismatch = the smallest such bit pattern (0xffffffff if none). */
 int pdf_check_exact_sequence(int which, uint32_t lo, uint32_t hi, uint64_t *mismatches, uint32_t *first_mismatch);
 int pdf_device_count(int *count);

@@ -67,6 +67,17 @@ struct BatchItem {
     unsigned item, block;
 };
 
+// Plane pointers of the batched kernels come from argument tables in device memory, so the
+// compiler cannot tell that they address global memory and would access them with flat
+// instructions.  Those count against lgkmcnt as well as vmcnt, so every wait for a scalar load
+// or an LDS access would also drain the loads in flight.  gptr() states the address space, and
+// the accesses become global loads and stores with a scalar base.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gptr(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2): block b runs
 // on XCD b % 8.  xcd_swizzle maps dispatch order to a tile order in which every XCD walks one
 // contiguous run of tiles, so neighbouring tiles (whose regions share halo rows and cache

@@ -44,7 +44,7 @@ __device__ __forceinline__ void pd_fused_body(const PDArgs &a, unsigned bx, unsi
     const int v0 = (int)bx * TV, u0 = (int)by * TU;
     const int rows = (int)a.rows, cols = (int)a.cols;
     const float li = a.lambda_i, ld = a.lambda_d;
-    const float *du_acc = a.in[S_DU_ACC], *dv_acc = a.in[S_DV_ACC], *dw_acc = a.in[S_DW_ACC];
+    const auto du_acc = gptr(a.in[S_DU_ACC]), dv_acc = gptr(a.in[S_DV_ACC]), dw_acc = gptr(a.in[S_DW_ACC]);
 
     // own pixel: issue every global load before the first barrier
     const int gv = v0 + tv, gu = u0 + tu;
@@ -55,11 +55,11 @@ __device__ __forceinline__ void pd_fused_body(const PDArgs &a, unsigned bx, unsi
     float p_in[6] = {0, 0, 0, 0, 0, 0};
     if (inside) {
         ua = du_acc[g]; va = dv_acc[g]; wa = dw_acc[g];
-        du0 = a.in[S_DU][g]; dv0 = a.in[S_DV][g]; dw0 = a.in[S_DW][g]; pd_in = a.in[S_PD][g];
+        du0 = gptr(a.in[S_DU])[g]; dv0 = gptr(a.in[S_DV])[g]; dw0 = gptr(a.in[S_DW])[g]; pd_in = gptr(a.in[S_PD])[g];
 #pragma unroll
-        for (int k = 0; k < 6; k++) p_in[k] = a.in[S_PUU + k][g];
-        mu_uv = a.mu_uv[g]; ddu = a.ddu[g]; ddv = a.ddv[g]; ddt = a.ddt[g];
-        dct = a.dct[g]; dcu = a.dcu[g]; dcv = a.dcv[g];
+        for (int k = 0; k < 6; k++) p_in[k] = gptr(a.in[S_PUU + k])[g];
+        mu_uv = gptr(a.mu_uv)[g]; ddu = gptr(a.ddu)[g]; ddv = gptr(a.ddv)[g]; ddt = gptr(a.ddt)[g];
+        dct = gptr(a.dct)[g]; dcu = gptr(a.dcu)[g]; dcv = gptr(a.dcv)[g];
     }
 
     // stage (acc + prev) on the ring region [v0-1, v0+TV] x [u0-1, u0+TU]
@@ -69,8 +69,8 @@ __device__ __forceinline__ void pd_fused_body(const PDArgs &a, unsigned bx, unsi
         float su = 0.f, sv = 0.f, sw = 0.f;
         if (qv >= 0 && qv < rows && qu >= 0 && qu < cols) {
             const unsigned q = (unsigned)qv + (unsigned)qu * a.rows;
-            su = du_acc[q] + a.du_prev[q];
-            sv = dv_acc[q] + a.dv_prev[q];
+            su = du_acc[q] + gptr(a.du_prev)[q];
+            sv = dv_acc[q] + gptr(a.dv_prev)[q];
             sw = dw_acc[q];
         }
         sU[e] = su; sV[e] = sv; sW[e] = sw;
@@ -82,8 +82,8 @@ __device__ __forceinline__ void pd_fused_body(const PDArgs &a, unsigned bx, unsi
         float ri = 0.f, rj = 0.f;
         if (qv >= 0 && qv < rows && qu >= 0 && qu < cols) {
             const unsigned q = (unsigned)qv + (unsigned)qu * a.rows;
-            ri = a.ri[q];
-            rj = a.rj[q];
+            ri = gptr(a.ri)[q];
+            rj = gptr(a.rj)[q];
         }
         sRi[e] = ri; sRj[e] = rj;
     }
@@ -99,7 +99,7 @@ __device__ __forceinline__ void pd_fused_body(const PDArgs &a, unsigned bx, unsi
         if (halo) {
             const unsigned hg = (unsigned)hv + (unsigned)hu * a.rows;
 #pragma unroll
-            for (int k = 0; k < 6; k++) hp[k] = a.in[S_PUU + k][hg];
+            for (int k = 0; k < 6; k++) hp[k] = gptr(a.in[S_PUU + k])[hg];
         }
     }
     __syncthreads();
@@ -145,11 +145,11 @@ __device__ __forceinline__ void pd_fused_body(const PDArgs &a, unsigned bx, unsi
     if (inside) {
         const TVDual p = tv_dual(tv, tu, gv, gu, p_in);
         store_dual(h, p);
-        a.out[S_PUU][g] = p.puu; a.out[S_PUV][g] = p.puv; a.out[S_PVU][g] = p.pvu;
-        a.out[S_PVV][g] = p.pvv; a.out[S_PWU][g] = p.pwu; a.out[S_PWV][g] = p.pwv;
+        gptr(a.out[S_PUU])[g] = p.puu; gptr(a.out[S_PUV])[g] = p.puv; gptr(a.out[S_PVU])[g] = p.pvu;
+        gptr(a.out[S_PVV])[g] = p.pvv; gptr(a.out[S_PWU])[g] = p.pwu; gptr(a.out[S_PWV])[g] = p.pwv;
         pd = pd_in + sigma_pd_of(mu_uv, ddu, ddv) * mu_uv * (-wa + ddt + ddu * ua + ddv * va);
         pd = clamp_pd(pd);
-        a.out[S_PD][g] = pd;
+        gptr(a.out[S_PD])[g] = pd;
     }
     if (halo) store_dual((heu + 1) * HV + (hev + 1), tv_dual(hev, heu, v0 + hev, u0 + heu, hp));
     __syncthreads();
@@ -170,12 +170,12 @@ __device__ __forceinline__ void pd_fused_body(const PDArgs &a, unsigned bx, unsi
         const Primal p = primal_update(du0, dv0, dw0, tau_uv_of(mu_uv, ddu, li, acu_r),
                                        tau_uv_of(mu_uv, ddv, li, acu_r), tau_w_of(mu_uv, ld, acu_r),
                                        mu_uv, pd, ddu, ddv, divu, divv, divw, dct, dcu, dcv, li, ld);
-        a.out[S_DU_ACC][g] = 2.f * p.du - du0;
-        a.out[S_DV_ACC][g] = 2.f * p.dv - dv0;
-        a.out[S_DW_ACC][g] = 2.f * p.dw - dw0;
-        a.out[S_DU][g] = p.du;
-        a.out[S_DV][g] = p.dv;
-        a.out[S_DW][g] = p.dw;
+        gptr(a.out[S_DU_ACC])[g] = 2.f * p.du - du0;
+        gptr(a.out[S_DV_ACC])[g] = 2.f * p.dv - dv0;
+        gptr(a.out[S_DW_ACC])[g] = 2.f * p.dw - dw0;
+        gptr(a.out[S_DU])[g] = p.du;
+        gptr(a.out[S_DV])[g] = p.dv;
+        gptr(a.out[S_DW])[g] = p.dw;
     }
 }
 
@@ -207,12 +207,12 @@ __device__ __forceinline__ void load_state(Px &q, const PDArgs &a, int gv, int g
     q.gu = gu;
     q.on = on;
     const unsigned g = px_index(a, gv, gu, on);
-    q.du = a.in[S_DU][g]; q.dv = a.in[S_DV][g]; q.dw = a.in[S_DW][g];
-    q.ua = a.in[S_DU_ACC][g]; q.va = a.in[S_DV_ACC][g]; q.wa = a.in[S_DW_ACC][g];
-    q.pd = a.in[S_PD][g];
+    q.du = gptr(a.in[S_DU])[g]; q.dv = gptr(a.in[S_DV])[g]; q.dw = gptr(a.in[S_DW])[g];
+    q.ua = gptr(a.in[S_DU_ACC])[g]; q.va = gptr(a.in[S_DV_ACC])[g]; q.wa = gptr(a.in[S_DW_ACC])[g];
+    q.pd = gptr(a.in[S_PD])[g];
 #pragma unroll
-    for (int k = 0; k < 6; k++) q.p[k] = a.in[S_PUU + k][g];
-    q.ri = a.ri[g]; q.rj = a.rj[g];
+    for (int k = 0; k < 6; k++) q.p[k] = gptr(a.in[S_PUU + k])[g];
+    q.ri = gptr(a.ri)[g]; q.rj = gptr(a.rj)[g];
 }
 
 // the other 9 per-level constant planes
@@ -235,18 +235,18 @@ __device__ __forceinline__ void load_px(Px &q, const PDArgs &a, int gv, int gu, 
     const unsigned g = px_index(a, gv, gu, on);
     float c[kConstPlanes];
 #pragma unroll
-    for (int k = 0; k < kConstPlanes; k++) c[k] = const_plane(a, k)[g];
+    for (int k = 0; k < kConstPlanes; k++) c[k] = gptr(const_plane(a, k))[g];
     set_consts(q, c);
 }
 
 __device__ __forceinline__ void store_px(const Px &q, const PDArgs &a)
 {
     const unsigned g = (unsigned)q.gv + (unsigned)q.gu * a.rows;
-    a.out[S_DU][g] = q.du; a.out[S_DV][g] = q.dv; a.out[S_DW][g] = q.dw;
-    a.out[S_DU_ACC][g] = q.ua; a.out[S_DV_ACC][g] = q.va; a.out[S_DW_ACC][g] = q.wa;
-    a.out[S_PD][g] = q.pd;
+    gptr(a.out[S_DU])[g] = q.du; gptr(a.out[S_DV])[g] = q.dv; gptr(a.out[S_DW])[g] = q.dw;
+    gptr(a.out[S_DU_ACC])[g] = q.ua; gptr(a.out[S_DV_ACC])[g] = q.va; gptr(a.out[S_DW_ACC])[g] = q.wa;
+    gptr(a.out[S_PD])[g] = q.pd;
 #pragma unroll
-    for (int k = 0; k < 6; k++) a.out[S_PUU + k][g] = q.p[k];
+    for (int k = 0; k < 6; k++) gptr(a.out[S_PUU + k])[g] = q.p[k];
 }
 
 // riu = ri at (v, u-1), rjv = rj at (v-1, u); only read when those pixels exist

@@ -549,13 +549,13 @@ __global__ void __launch_bounds__(256) k_check_exact(int which, unsigned lo, uns
         const float x = __uint_as_float(bits);
         float fast, ieee;
         if (which == 0) {
-            fast = sqrt_rn_gt1(x);
+            fast = sqrt_rn_ge1(x);
             ieee = sqrtf(x);
         } else if (which == 1) {
             fast = rcp_rn_ge1(x);
             ieee = 1.f / x;
         } else if (which == 2) {
-            fast = inv_sqrt_rn_gt1(x);
+            fast = inv_sqrt_rn_ge1(x);
             ieee = 1.f / sqrtf(x);
         } else if (which == 3) {
             fast = div_rn_half_or_one(1.f, x);

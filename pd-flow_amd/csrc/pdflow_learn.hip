@@ -56,8 +56,8 @@ __global__ void __launch_bounds__(kLearnBlock) kb_gather_inputs(BatchTable t)
     const unsigned i = it.block * kLearnBlock + threadIdx.x;     // row-major pixel
     if (i >= a.rows * a.cols) return;
     const unsigned v = i / a.cols, u = i % a.cols, c = v + u * a.rows;
-    const float4 x = make_float4(a.I1[c] * (1.f / 255.f) - 0.5f, a.Z1[c] * 0.25f,
-                                 a.I2[c] * (1.f / 255.f) - 0.5f, a.Z2[c] * 0.25f);
+    const float4 x = make_float4(gptr(a.I1)[c] * (1.f / 255.f) - 0.5f, gptr(a.Z1)[c] * 0.25f,
+                                 gptr(a.I2)[c] * (1.f / 255.f) - 0.5f, gptr(a.Z2)[c] * 0.25f);
     reinterpret_cast<float4 *>(a.x)[i] = x;
 }
 
@@ -69,9 +69,9 @@ __global__ void __launch_bounds__(kLearnBlock) kb_inject(BatchTable t)
     const unsigned i = it.block * kLearnBlock + threadIdx.x;
     if (i >= P) return;
     const unsigned v = i / a.cols, u = i % a.cols, c = v + u * a.rows;
-    a.warm[c] = a.y[3 * i];
-    a.warm[P + c] = a.y[3 * i + 1];
-    a.warm[2 * P + c] = a.y[3 * i + 2] / 10.f;
+    gptr(a.warm)[c] = gptr(a.y)[3 * i];
+    gptr(a.warm)[P + c] = gptr(a.y)[3 * i + 1];
+    gptr(a.warm)[2 * P + c] = gptr(a.y)[3 * i + 2] / 10.f;
 }
 
 __global__ void __launch_bounds__(kLearnBlock) kb_labels(BatchTable t)
@@ -86,15 +86,15 @@ __global__ void __launch_bounds__(kLearnBlock) kb_labels(BatchTable t)
     for (unsigned du = 0; du < s; du++)
         for (unsigned dv = 0; dv < s; dv++) {
             const unsigned c = (v * s + dv) + (u * s + du) * a.frows;
-            su += a.du[c];
-            sv += a.dv[c];
-            sw += a.dw[c];
+            su += gptr(a.du)[c];
+            sv += gptr(a.dv)[c];
+            sw += gptr(a.dw)[c];
         }
     const float inv = 1.f / (float)(s * s);
-    a.t[3 * i] = su * inv / (float)s;
-    a.t[3 * i + 1] = sv * inv / (float)s;
-    a.t[3 * i + 2] = sw * inv * 10.f;
-    a.m[i] = 1.f;
+    gptr(a.t)[3 * i] = su * inv / (float)s;
+    gptr(a.t)[3 * i + 1] = sv * inv / (float)s;
+    gptr(a.t)[3 * i + 2] = sw * inv * 10.f;
+    gptr(a.m)[i] = 1.f;
 }
 
 __global__ void __launch_bounds__(kLearnBlock) kb_metrics(BatchTable t, float *partial)
@@ -106,30 +106,30 @@ __global__ void __launch_bounds__(kLearnBlock) kb_metrics(BatchTable t, float *p
     float acc[3] = {0.f, 0.f, 0.f};
     if (idx < rows * cols) {
         const unsigned v = idx % rows, u = idx / rows;
-        const float du = a.du[idx], dv = a.dv[idx], dw = a.dw[idx];
-        const float ub = du + a.du_prev[idx], vb = dv + a.dv_prev[idx];
-        const float ri = a.ri[idx], rj = a.rj[idx];
+        const float du = gptr(a.du)[idx], dv = gptr(a.dv)[idx], dw = gptr(a.dw)[idx];
+        const float ub = du + gptr(a.du_prev)[idx], vb = dv + gptr(a.dv_prev)[idx];
+        const float ri = gptr(a.ri)[idx], rj = gptr(a.rj)[idx];
         float gu1 = 0.f, gv1 = 0.f, gw1 = 0.f, gu2 = 0.f, gv2 = 0.f, gw2 = 0.f;
         if (u + 1 < cols) {
             const unsigned n = idx + rows;
-            gu1 = ri * ((a.du[n] + a.du_prev[n]) - ub);
-            gv1 = ri * ((a.dv[n] + a.dv_prev[n]) - vb);
-            gw1 = ri * (a.dw[n] - dw);
+            gu1 = ri * ((gptr(a.du)[n] + gptr(a.du_prev)[n]) - ub);
+            gv1 = ri * ((gptr(a.dv)[n] + gptr(a.dv_prev)[n]) - vb);
+            gw1 = ri * (gptr(a.dw)[n] - dw);
         }
         if (v + 1 < rows) {
             const unsigned n = idx + 1;
-            gu2 = rj * ((a.du[n] + a.du_prev[n]) - ub);
-            gv2 = rj * ((a.dv[n] + a.dv_prev[n]) - vb);
-            gw2 = rj * (a.dw[n] - dw);
+            gu2 = rj * ((gptr(a.du)[n] + gptr(a.du_prev)[n]) - ub);
+            gv2 = rj * ((gptr(a.dv)[n] + gptr(a.dv_prev)[n]) - vb);
+            gw2 = rj * (gptr(a.dw)[n] - dw);
         }
-        const float data_i = fabsf(a.dct[idx] + a.dcu[idx] * du + a.dcv[idx] * dv);
-        const float data_d = a.mu_uv[idx] * fabsf(a.ddt[idx] + a.ddu[idx] * du + a.ddv[idx] * dv - dw);
+        const float data_i = fabsf(gptr(a.dct)[idx] + gptr(a.dcu)[idx] * du + gptr(a.dcv)[idx] * dv);
+        const float data_d = gptr(a.mu_uv)[idx] * fabsf(gptr(a.ddt)[idx] + gptr(a.ddu)[idx] * du + gptr(a.ddv)[idx] * dv - dw);
         const float tv = a.lambda_i * (sqrtf(gu1 * gu1 + gu2 * gu2) + sqrtf(gv1 * gv1 + gv2 * gv2)) +
                          a.lambda_d * sqrtf(gw1 * gw1 + gw2 * gw2);
         acc[0] = data_i + data_d + tv;
-        if (a.Zold[idx] > 0.f) {
-            const Tap tp = make_tap(rows, cols, (float)u + a.du_l[idx], (float)v + a.dv_l[idx]);
-            acc[1] = fabsf(bilinear(a.I2, rows, tp) - a.I1[idx]);
+        if (gptr(a.Zold)[idx] > 0.f) {
+            const Tap tp = make_tap(rows, cols, (float)u + gptr(a.du_l)[idx], (float)v + gptr(a.dv_l)[idx]);
+            acc[1] = fabsf(bilinear(a.I2, rows, tp) - gptr(a.I1)[idx]);
             acc[2] = 1.f;
         }
     }
@@ -142,8 +142,8 @@ __global__ void __launch_bounds__(kLearnBlock) kb_epe(BatchTable t, float *parti
     const EpeArgs &a = static_cast<const EpeArgs *>(t.args)[it.item];
     const unsigned i = it.block * kLearnBlock + threadIdx.x;
     float acc[2] = {0.f, 0.f};
-    if (i < a.pixels && a.zold[i] > 0.f) {
-        const float ex = a.ax[i] - a.bx[i], ey = a.ay[i] - a.by[i], ez = a.az[i] - a.bz[i];
+    if (i < a.pixels && gptr(a.zold)[i] > 0.f) {
+        const float ex = gptr(a.ax)[i] - gptr(a.bx)[i], ey = gptr(a.ay)[i] - gptr(a.by)[i], ez = gptr(a.az)[i] - gptr(a.bz)[i];
         acc[0] = sqrtf(ex * ex + ey * ey + ez * ez);
         acc[1] = 1.f;
     }

@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "pdflow_internal.hpp"
+
 namespace pdf {
 
 __device__ __forceinline__ float rhyp(float a, float b) { return 1.f / sqrtf(a * a + b * b); }
@@ -19,9 +21,9 @@ __device__ __forceinline__ float rhyp(float a, float b) { return 1.f / sqrtf(a *
 // tests/test_gpu_exact.py sweeps every float of each range against sqrtf and "/" and finds the
 // same bits everywhere.
 
-// sqrtf(x) for 1 < x <= FLT_MAX (and +inf): the estimate s, then the neighbour of s whose
-// residual x - s'*s shows that s was one step off (one below, one above).
-__device__ __forceinline__ float sqrt_rn_gt1(float x)
+// sqrtf(x) for 1 <= x <= +inf: the estimate s, then the neighbour of s whose residual
+// x - s'*s shows that s was one step off (one below, one above).
+__device__ __forceinline__ float sqrt_rn_ge1(float x)
 {
     const float s = __builtin_amdgcn_sqrtf(x);
     const float below = __int_as_float(__float_as_int(s) - 1);
@@ -55,10 +57,12 @@ __device__ __forceinline__ float div_half_or_one(float n, float d)
     return n / d;
 }
 
-// 1.f / sqrtf(x) for x > 1 (1/+inf = 0)
-__device__ __forceinline__ float inv_sqrt_rn_gt1(float x)
+// 1.f / sqrtf(x) for x >= 1 (1/+inf = 0).  The sequence is evaluated for every x and the
+// result selected, so that callers stay free of branches.
+__device__ __forceinline__ float inv_sqrt_rn_ge1(float x)
 {
-    return x < __builtin_inff() ? rcp_rn_ge1(sqrt_rn_gt1(x)) : 0.f;
+    const float m = rcp_rn_ge1(sqrt_rn_ge1(x));
+    return x < __builtin_inff() ? m : 0.f;
 }
 
 
@@ -82,13 +86,13 @@ __device__ __forceinline__ float tau_w_of(float mu_uv, float lam, float acu_r)
 __device__ __forceinline__ void project(float &a, float &b)
 {
     // unit-disc projection (.cu:849-859): m = 1/sqrt(a*a+b*b), scale when m < 1.  With
-    // correctly rounded sqrt and division, x = a*a+b*b <= 1 gives sqrt(x) <= 1 and so m >= 1:
-    // only x > 1 can scale, with the same result for every input (NaN compares false either
-    // way).  m is evaluated for every x and selected, so no lane branches; for x <= 1 it is
-    // outside inv_sqrt_rn_gt1's range and unused.
+    // correctly rounded sqrt and division, x = a*a+b*b <= 1 gives sqrt(x) <= 1 and so m >= 1,
+    // and a NaN x gives a NaN m: neither scales.  Here x is raised to 1 first (fmaxf returns 1
+    // for a NaN x), which gives m = 1 in exactly those cases, and the reference's m for every
+    // x > 1, +inf included.
     const float x = a * a + b * b;
-    const float m = inv_sqrt_rn_gt1(x);
-    const bool scale = x > 1.f && m < 1.f;
+    const float m = inv_sqrt_rn_ge1(fmaxf(x, 1.f));
+    const bool scale = m < 1.f;
     a = scale ? a * m : a;
     b = scale ? b * m : b;
 }
@@ -171,9 +175,11 @@ __device__ __forceinline__ Tap make_tap(unsigned rows_i, unsigned cols_i, float 
     return t;
 }
 
-__device__ __forceinline__ float bilinear(const float *mat, unsigned rows_i, const Tap &t)
+// mat is a plane in global memory
+__device__ __forceinline__ float bilinear(const float *plane, unsigned rows_i, const Tap &t)
 {
     // interpolatePixel (.cu:666-681) after clamping
+    const auto mat = gptr(plane);
     if (t.su == t.fu && t.sv == t.fv) return mat[(unsigned)rintf(t.iv + (float)rows_i * t.iu)];
     if (t.su == t.fu) {
         const unsigned cu = (unsigned)rintf(t.iu);
@@ -188,15 +194,16 @@ __device__ __forceinline__ float bilinear(const float *mat, unsigned rows_i, con
     return ((float)t.sv - t.iv) * bot + (t.iv - (float)t.fv) * top;
 }
 
-__device__ __forceinline__ float bilinear_depth(const float *mat, unsigned rows_i, const Tap &t)
+__device__ __forceinline__ float bilinear_depth(const float *plane, unsigned rows_i, const Tap &t)
 {
     // interpolatePixelDepth (.cu:696-720): nearest neighbour (half to even) if a tap is 0
+    const auto mat = gptr(plane);
     if (mat[t.sv + rows_i * t.su] == 0.f || mat[t.sv + rows_i * t.fu] == 0.f ||
         mat[t.fv + rows_i * t.su] == 0.f || mat[t.fv + rows_i * t.fu] == 0.f) {
         const unsigned ru = (unsigned)rintf(t.iu), rv = (unsigned)rintf(t.iv);
         return mat[rv + rows_i * ru];
     }
-    return bilinear(mat, rows_i, t);
+    return bilinear(plane, rows_i, t);
 }
 
 // Stable sort of 9 (key, weight) pairs by key.  The reference uses a bubble sort with a

@@ -41,11 +41,14 @@ __host__ __device__ inline unsigned tiles_of(unsigned rows, unsigned cols)
     return ((rows + kTR - 1) / kTR) * ((cols + kTC - 1) / kTC);
 }
 
+// the coarse patch a tile's 5x5 even taps touch: coarse rows v0/2-1 .. v0/2+16, columns
+// u0/2-1 .. u0/2+4 (clamped to the coarse level), at most 18 x 6, stored with row stride kCR
+constexpr int kCR = kTR / 2 + 3, kCC = kTC / 2 + 3;
+static_assert(kCR * kCC <= kBlock, "one coarse element per thread and plane");
+
 __device__ __forceinline__ void level_init_body(const LevelArgs &a, unsigned tile, int has_coarse)
 {
-    // the coarse patch a tile's 5x5 even taps touch: coarse rows v0/2-1 .. v0/2+16, columns
-    // u0/2-1 .. u0/2+4 (clamped to the coarse level)
-    __shared__ float sC[10][19 * 7];
+    __shared__ float sC[10][kCR * kCC];
     const unsigned rows_i = a.rows, cols_i = a.cols;
     const unsigned tv = (rows_i + kTR - 1) / kTR;
     const int v0 = (int)(tile % tv) * kTR, u0 = (int)(tile / tv) * kTC;
@@ -59,10 +62,11 @@ __device__ __forceinline__ void level_init_body(const LevelArgs &a, unsigned til
         const int rc = (int)rows_i / 2, cc = (int)cols_i / 2;
         const int cr0 = v0 >= 2 ? (v0 - 2) / 2 : 0, cr1 = min(rc - 1, (v0 + kTR + 1) / 2);
         const int cc0 = u0 >= 2 ? (u0 - 2) / 2 : 0, cc1 = min(cc - 1, (u0 + kTC + 1) / 2);
-        const int CR = cr1 - cr0 + 1, CC = cc1 - cc0 + 1;
-        for (int e = (int)threadIdx.x; e < 10 * CR * CC; e += kBlock) {
-            const int q = e / (CR * CC), r = e - q * (CR * CC);
-            sC[q][r] = a.f[kCarry[q]][(unsigned)(cr0 + r % CR) + (unsigned)(cc0 + r / CR) * (unsigned)rc];
+        const int t = (int)threadIdx.x, tr = t % kCR, tc = t / kCR;
+        if (t < kCR * kCC && tr <= cr1 - cr0 && tc <= cc1 - cc0) {
+            const unsigned src = (unsigned)(cr0 + tr) + (unsigned)(cc0 + tc) * (unsigned)rc;
+#pragma unroll
+            for (int q = 0; q < 10; q++) sC[q][t] = gptr(a.f[kCarry[q]])[src];
         }
         __syncthreads();
         if (on) {
@@ -78,7 +82,7 @@ __device__ __forceinline__ void level_init_body(const LevelArgs &a, unsigned til
                     const int uu = u + l;
                     if (uu < 0 || uu >= (int)cols_i) continue;
                     const float g = 4.f * a.g[12 + k + 5 * l];
-                    const int ci = (vv / 2 - cr0) + (uu / 2 - cc0) * CR;
+                    const int ci = (vv / 2 - cr0) + (uu / 2 - cc0) * kCR;
 #pragma unroll
                     for (int q = 0; q < 10; q++) {
                         const float cval = sC[q][ci];
@@ -106,10 +110,10 @@ __device__ __forceinline__ void level_init_body(const LevelArgs &a, unsigned til
     if (!on) return;
     // .cu:447-454: du_new, dv_new, du_acc, dv_acc start at 0 on every level
 #pragma unroll
-    for (int k = 0; k < 4; k++) a.f[kInitZero[k]][index] = 0.f;
+    for (int k = 0; k < 4; k++) gptr(a.f[kInitZero[k]])[index] = 0.f;
 #pragma unroll
-    for (int q = 0; q < 10; q++) a.f[kUpDst[q]][index] = acc[q];
-    a.f[PDF_DW_ACC][index] = acc[2];
+    for (int q = 0; q < 10; q++) gptr(a.f[kUpDst[q]])[index] = acc[q];
+    gptr(a.f[PDF_DW_ACC])[index] = acc[2];
 }
 
 // ------------------------------------------------------------------ warm start
@@ -121,15 +125,15 @@ __device__ __forceinline__ void warm_init_body(const LevelArgs &a, unsigned bloc
     const unsigned index = block * kBlock + threadIdx.x;
     const unsigned P = a.rows * a.cols;
     if (index >= P) return;
-    const float du = a.warm[index], dv = a.warm[P + index], dw = a.warm[2 * P + index];
+    const float du = gptr(a.warm)[index], dv = gptr(a.warm)[P + index], dw = gptr(a.warm)[2 * P + index];
 #pragma unroll
-    for (int k = 0; k < 4; k++) a.f[kInitZero[k]][index] = 0.f;
-    a.f[PDF_DU_PREV][index] = du;
-    a.f[PDF_DV_PREV][index] = dv;
-    a.f[PDF_DW_NEW][index] = dw;
-    a.f[PDF_DW_ACC][index] = dw;
+    for (int k = 0; k < 4; k++) gptr(a.f[kInitZero[k]])[index] = 0.f;
+    gptr(a.f[PDF_DU_PREV])[index] = du;
+    gptr(a.f[PDF_DV_PREV])[index] = dv;
+    gptr(a.f[PDF_DW_NEW])[index] = dw;
+    gptr(a.f[PDF_DW_ACC])[index] = dw;
 #pragma unroll
-    for (int q = 3; q < 10; q++) a.f[kUpDst[q]][index] = 0.f;   // pd and the six TV duals
+    for (int q = 3; q < 10; q++) gptr(a.f[kUpDst[q]])[index] = 0.f;   // pd and the six TV duals
 }
 
 // ------------------------------------------------------------------ K5 + K6
@@ -148,25 +152,25 @@ __device__ __forceinline__ void rij_gradients_body(const LevelArgs &a, unsigned 
     float dxu = 0.f, dzu = 0.f, dxu2 = 0.f, dzu2 = 0.f;
     float dyv = 0.f, dzv = 0.f, dyv2 = 0.f, dzv2 = 0.f;
     if (u != cols_i - 1) {
-        dxu = a.Xold[index + r] - a.Xold[index];
-        dzu = a.Zold[index + r] - a.Zold[index];
-        dxu2 = a.X[index + r] - a.X[index];
-        dzu2 = a.Z[index + r] - a.Z[index];
+        dxu = gptr(a.Xold)[index + r] - gptr(a.Xold)[index];
+        dzu = gptr(a.Zold)[index + r] - gptr(a.Zold)[index];
+        dxu2 = gptr(a.X)[index + r] - gptr(a.X)[index];
+        dzu2 = gptr(a.Z)[index + r] - gptr(a.Z)[index];
     }
     if (v != rows_i - 1) {
-        dyv = a.Yold[index + 1] - a.Yold[index];
-        dzv = a.Zold[index + 1] - a.Zold[index];
-        dyv2 = a.Y[index + 1] - a.Y[index];
-        dzv2 = a.Z[index + 1] - a.Z[index];
+        dyv = gptr(a.Yold)[index + 1] - gptr(a.Yold)[index];
+        dzv = gptr(a.Zold)[index + 1] - gptr(a.Zold)[index];
+        dyv2 = gptr(a.Y)[index + 1] - gptr(a.Y)[index];
+        dzv2 = gptr(a.Z)[index + 1] - gptr(a.Z)[index];
     }
     const float ri2 = rweight(dxu2, dzu2), rj2 = rweight(dyv2, dzv2);
-    a.f[PDF_RI][index] = rweight(dxu, dzu);
-    a.f[PDF_RJ][index] = rweight(dyv, dzv);
-    a.f[PDF_RI_2][index] = ri2;
-    a.f[PDF_RJ_2][index] = rj2;
+    gptr(a.f[PDF_RI])[index] = rweight(dxu, dzu);
+    gptr(a.f[PDF_RJ])[index] = rweight(dyv, dzv);
+    gptr(a.f[PDF_RI_2])[index] = ri2;
+    gptr(a.f[PDF_RJ_2])[index] = rj2;
     // K6 (.cu:574-627); ri_2 at (v, u-1) and rj_2 at (v-1, u) recomputed (u-1 and v-1 are
     // never the last column/row there, so their differences always exist)
-    const float *I = a.I, *Z = a.Z;
+    const auto I = gptr(a.I), Z = gptr(a.Z);
     float dcu, ddu, dcv, ddv;
     if (u == 0) {
         dcu = I[index + r] - I[index];
@@ -175,7 +179,7 @@ __device__ __forceinline__ void rij_gradients_body(const LevelArgs &a, unsigned 
         dcu = I[index] - I[index - r];
         ddu = Z[index] - Z[index - r];
     } else {
-        const float rp = rweight(a.X[index] - a.X[index - r], Z[index] - Z[index - r]);
+        const float rp = rweight(gptr(a.X)[index] - gptr(a.X)[index - r], Z[index] - Z[index - r]);
         dcu = wdiff(ri2, rp, I[index + r], I[index], I[index - r]);
         ddu = (Z[index] > 0.f) ? wdiff(ri2, rp, Z[index + r], Z[index], Z[index - r]) : 0.f;
     }
@@ -186,14 +190,14 @@ __device__ __forceinline__ void rij_gradients_body(const LevelArgs &a, unsigned 
         dcv = I[index] - I[index - 1];
         ddv = Z[index] - Z[index - 1];
     } else {
-        const float rp = rweight(a.Y[index] - a.Y[index - 1], Z[index] - Z[index - 1]);
+        const float rp = rweight(gptr(a.Y)[index] - gptr(a.Y)[index - 1], Z[index] - Z[index - 1]);
         dcv = wdiff(rj2, rp, I[index + 1], I[index], I[index - 1]);
         ddv = (Z[index] > 0.f) ? wdiff(rj2, rp, Z[index + 1], Z[index], Z[index - 1]) : 0.f;
     }
-    a.f[PDF_DCU_AUX][index] = dcu;
-    a.f[PDF_DDU_AUX][index] = ddu;
-    a.f[PDF_DCV_AUX][index] = dcv;
-    a.f[PDF_DDV_AUX][index] = ddv;
+    gptr(a.f[PDF_DCU_AUX])[index] = dcu;
+    gptr(a.f[PDF_DDU_AUX])[index] = ddu;
+    gptr(a.f[PDF_DCV_AUX])[index] = dcv;
+    gptr(a.f[PDF_DDV_AUX])[index] = ddv;
 }
 
 // ------------------------------------------------------------------ K7 + K8
@@ -204,39 +208,39 @@ __device__ __forceinline__ void warp_mu_body(const LevelArgs &a, unsigned block,
     if (index >= rows_i * cols_i) return;
     const unsigned v = index % rows_i, u = index / rows_i;
     // K7 (.cu:629-652)
-    const float iu = (float)u + a.f[PDF_DU_PREV][index];
-    const float iv = (float)v + a.f[PDF_DV_PREV][index];
+    const float iu = (float)u + gptr(a.f[PDF_DU_PREV])[index];
+    const float iv = (float)v + gptr(a.f[PDF_DV_PREV])[index];
     const Tap t = make_tap(rows_i, cols_i, iu, iv);
-    const float dct = bilinear(a.I, rows_i, t) - a.Iold[index];
+    const float dct = bilinear(a.I, rows_i, t) - gptr(a.Iold)[index];
     const float dcu = bilinear(a.f[PDF_DCU_AUX], rows_i, t);
     const float dcv = bilinear(a.f[PDF_DCV_AUX], rows_i, t);
     const float wd = bilinear_depth(a.Z, rows_i, t);
-    const float ddt = (wd > 0.f) ? wd - a.Zold[index] : 0.f;
+    const float ddt = (wd > 0.f) ? wd - gptr(a.Zold)[index] : 0.f;
     const float ddu = bilinear(a.f[PDF_DDU_AUX], rows_i, t);
     const float ddv = bilinear(a.f[PDF_DDV_AUX], rows_i, t);
-    a.f[PDF_DCT][index] = dct;
-    a.f[PDF_DCU][index] = dcu;
-    a.f[PDF_DCV][index] = dcv;
-    a.f[PDF_DDT][index] = ddt;
-    a.f[PDF_DDU][index] = ddu;
-    a.f[PDF_DDV][index] = ddv;
+    gptr(a.f[PDF_DCT])[index] = dct;
+    gptr(a.f[PDF_DCU])[index] = dcu;
+    gptr(a.f[PDF_DCV])[index] = dcv;
+    gptr(a.f[PDF_DDT])[index] = ddt;
+    gptr(a.f[PDF_DDU])[index] = ddu;
+    gptr(a.f[PDF_DDV])[index] = ddv;
     // K8 computeMu (.cu:781-784)
     const float mu_uv = a.mu / (1.f + 1000.f * (ddu * ddu + ddv * ddv + ddt * ddt));
-    a.f[PDF_MU_UV][index] = mu_uv;
+    gptr(a.f[PDF_MU_UV])[index] = mu_uv;
     if (write_steps) {
         // K8 computeStepSizes (.cu:786-808), needed only by the unfused iteration bridges
-        const float ri = a.f[PDF_RI][index], rj = a.f[PDF_RJ][index];
-        a.f[PDF_SIGMA_PD][index] = sigma_pd_of(mu_uv, ddu, ddv);
-        a.f[PDF_SIGMA_PUVX][index] = sigma_p_of(a.lambda_i, ri);
-        a.f[PDF_SIGMA_PUVY][index] = sigma_p_of(a.lambda_i, rj);
-        a.f[PDF_SIGMA_PWX][index] = 0.5f / (ri * a.lambda_d + 1e-10f);
-        a.f[PDF_SIGMA_PWY][index] = 0.5f / (rj * a.lambda_d + 1e-10f);
+        const float ri = gptr(a.f[PDF_RI])[index], rj = gptr(a.f[PDF_RJ])[index];
+        gptr(a.f[PDF_SIGMA_PD])[index] = sigma_pd_of(mu_uv, ddu, ddv);
+        gptr(a.f[PDF_SIGMA_PUVX])[index] = sigma_p_of(a.lambda_i, ri);
+        gptr(a.f[PDF_SIGMA_PUVY])[index] = sigma_p_of(a.lambda_i, rj);
+        gptr(a.f[PDF_SIGMA_PWX])[index] = 0.5f / (ri * a.lambda_d + 1e-10f);
+        gptr(a.f[PDF_SIGMA_PWY])[index] = 0.5f / (rj * a.lambda_d + 1e-10f);
         float acu_r = ri + rj;
-        if (u > 0) acu_r += a.f[PDF_RI][index - rows_i];
-        if (v > 0) acu_r += a.f[PDF_RJ][index - 1];
-        a.f[PDF_TAU_U][index] = tau_uv_of(mu_uv, ddu, a.lambda_i, acu_r);
-        a.f[PDF_TAU_V][index] = tau_uv_of(mu_uv, ddv, a.lambda_i, acu_r);
-        a.f[PDF_TAU_W][index] = tau_w_of(mu_uv, a.lambda_d, acu_r);
+        if (u > 0) acu_r += gptr(a.f[PDF_RI])[index - rows_i];
+        if (v > 0) acu_r += gptr(a.f[PDF_RJ])[index - 1];
+        gptr(a.f[PDF_TAU_U])[index] = tau_uv_of(mu_uv, ddu, a.lambda_i, acu_r);
+        gptr(a.f[PDF_TAU_V])[index] = tau_uv_of(mu_uv, ddv, a.lambda_i, acu_r);
+        gptr(a.f[PDF_TAU_W])[index] = tau_w_of(mu_uv, a.lambda_d, acu_r);
     }
 }
 
@@ -248,11 +252,11 @@ struct Sat {
 __device__ __forceinline__ Sat saturated(const LevelArgs &a, unsigned i)
 {
     // saturateVariables (.cu:1018-1038) of pixel i
-    float du = a.f[PDF_DU_NEW][i], dv = a.f[PDF_DV_NEW][i], dw = a.f[PDF_DW_NEW][i];
+    float du = gptr(a.f[PDF_DU_NEW])[i], dv = gptr(a.f[PDF_DV_NEW])[i], dw = gptr(a.f[PDF_DW_NEW])[i];
     if (du > 1.f) du = 1.f; else if (du < -1.f) du = -1.f;
     if (dv > 1.f) dv = 1.f; else if (dv < -1.f) dv = -1.f;
-    if (a.Zold[i] == 0.f) dw = 0.f;
-    return {du + a.f[PDF_DU_PREV][i], dv + a.f[PDF_DV_PREV][i], dw};
+    if (gptr(a.Zold)[i] == 0.f) dw = 0.f;
+    return {du + gptr(a.f[PDF_DU_PREV])[i], dv + gptr(a.f[PDF_DV_PREV])[i], dw};
 }
 
 // carry_duals: copy pd and the six duals into the *_l carry planes that the next level's
@@ -274,8 +278,8 @@ __device__ __forceinline__ void filter_motion_body(const LevelArgs &a, unsigned 
             sU[e] = f.du;
             sV[e] = f.dv;
             sW[e] = f.dw;
-            sZ[e] = a.Zold[il];
-            sT[e] = a.f[PDF_DDT][il];
+            sZ[e] = gptr(a.Zold)[il];
+            sT[e] = gptr(a.f[PDF_DDT])[il];
         }
     }
     __syncthreads();
@@ -331,74 +335,116 @@ __device__ __forceinline__ void filter_motion_body(const LevelArgs &a, unsigned 
     } else {
         ou = sU[hc]; ov = sV[hc]; ow = sW[hc];
     }
-    a.f[PDF_DU_L][index] = ou;
-    a.f[PDF_DV_L][index] = ov;
-    a.f[PDF_DW_L][index] = ow;
+    gptr(a.f[PDF_DU_L])[index] = ou;
+    gptr(a.f[PDF_DV_L])[index] = ov;
+    gptr(a.f[PDF_DW_L])[index] = ow;
     if (carry_duals) {
-        a.f[PDF_PD_L][index] = a.f[PDF_PD][index];
-        a.f[PDF_PUU_L][index] = a.f[PDF_PUU][index];
-        a.f[PDF_PUV_L][index] = a.f[PDF_PUV][index];
-        a.f[PDF_PVU_L][index] = a.f[PDF_PVU][index];
-        a.f[PDF_PVV_L][index] = a.f[PDF_PVV][index];
-        a.f[PDF_PWU_L][index] = a.f[PDF_PWU][index];
-        a.f[PDF_PWV_L][index] = a.f[PDF_PWV][index];
+        gptr(a.f[PDF_PD_L])[index] = gptr(a.f[PDF_PD])[index];
+        gptr(a.f[PDF_PUU_L])[index] = gptr(a.f[PDF_PUU])[index];
+        gptr(a.f[PDF_PUV_L])[index] = gptr(a.f[PDF_PUV])[index];
+        gptr(a.f[PDF_PVU_L])[index] = gptr(a.f[PDF_PVU])[index];
+        gptr(a.f[PDF_PVV_L])[index] = gptr(a.f[PDF_PVV])[index];
+        gptr(a.f[PDF_PWU_L])[index] = gptr(a.f[PDF_PWU])[index];
+        gptr(a.f[PDF_PWV_L])[index] = gptr(a.f[PDF_PWV])[index];
     }
     // K15 (.cu:1150-1167)
     if (d0 > 0.f) {
-        a.f[PDF_DX][index] = ow;
-        a.f[PDF_DY][index] = d0 * ou * a.inv_f + ow * a.Xold[index] / d0;
-        a.f[PDF_DZ][index] = d0 * ov * a.inv_f + ow * a.Yold[index] / d0;
+        gptr(a.f[PDF_DX])[index] = ow;
+        gptr(a.f[PDF_DY])[index] = d0 * ou * a.inv_f + ow * gptr(a.Xold)[index] / d0;
+        gptr(a.f[PDF_DZ])[index] = d0 * ov * a.inv_f + ow * gptr(a.Yold)[index] / d0;
     } else {
-        a.f[PDF_DX][index] = 0.f;
-        a.f[PDF_DY][index] = 0.f;
-        a.f[PDF_DZ][index] = 0.f;
+        gptr(a.f[PDF_DX])[index] = 0.f;
+        gptr(a.f[PDF_DY])[index] = 0.f;
+        gptr(a.f[PDF_DZ])[index] = 0.f;
     }
 }
 
 // ------------------------------------------------------------------ K1 for two frames
-__device__ __forceinline__ void pyramid_body(const PyrArgs &p, unsigned block)
+// One 32x8 tile of level rows_i x cols_i per block.  Above level 0 the tile's 5x5 neighbourhoods
+// in the finer level (rows 2*v0-2 .. 2*v0+64, columns 2*u0-2 .. 2*u0+16) are staged through LDS
+// first, by independent loads that run down the columns; the taps then read LDS.
+constexpr int kPR = 2 * kTR + 3, kPC = 2 * kTC + 3;   // 67 x 19 patch
+constexpr int kPatch = kPR * kPC;
+
+__device__ __forceinline__ void pyramid_body(const PyrArgs &p, unsigned tile)
 {
-    const unsigned index = block * kBlock + threadIdx.x;
+    __shared__ float sCol[kPatch], sDep[kPatch];
     const unsigned rows_i = p.rows_i, cols_i = p.cols_i;
-    if (index >= rows_i * cols_i) return;
-    const unsigned v = index % rows_i, u = index / rows_i;
+    const unsigned tv = (rows_i + kTR - 1) / kTR;
+    const int v0 = (int)(tile % tv) * kTR, u0 = (int)(tile / tv) * kTC;
+    const int v = v0 + (int)threadIdx.x % kTR, u = u0 + (int)threadIdx.x / kTR;
+    const bool on = v < (int)rows_i && u < (int)cols_i;
+    const unsigned index = (unsigned)v + (unsigned)u * rows_i;   // read only when on
     const float max_depth_dif = 0.1f;
     float depth;
     if (p.level == 0) {
-        p.col[index] = p.colour_wf[index];
-        depth = p.depth_wf[index];
-        p.dep[index] = depth;
+        if (!on) return;
+        gptr(p.col)[index] = gptr(p.colour_wf)[index];
+        depth = gptr(p.depth_wf)[index];
+        gptr(p.dep)[index] = depth;
     } else {
-        // .cu:357-424
-        float sumd = 0.f, sumc = 0.f, wd = 0.f, wc = 0.f;
-        const float dcen = p.depp[2 * v + 4 * u * rows_i];
-        const bool inner = (v > 0) && (v < rows_i - 1) && (u > 0) && (u < cols_i - 1);
+        const int pr = 2 * (int)rows_i, pc = 2 * (int)cols_i;   // the finer level
+        const int r0 = 2 * v0 - 2, c0 = 2 * u0 - 2;
 #pragma unroll
-        for (int k = -2; k < 3; k++)
-#pragma unroll
-            for (int l = -2; l < 3; l++) {
-                const int vv = 2 * (int)v + k, uu = 2 * (int)u + l;
-                if (!inner && !(vv >= 0 && vv < 2 * (int)rows_i && uu >= 0 && uu < 2 * (int)cols_i))
-                    continue;
-                const unsigned ip = (unsigned)vv + (unsigned)uu * 2 * rows_i;
-                const float g = p.g[12 + k + 5 * l];
-                sumc += g * p.colp[ip];
-                if (!inner) wc += g;
-                const float d = p.depp[ip];
-                if ((d > 0.f) && (fabsf(d - dcen) < max_depth_dif)) {
-                    const float w = g * (max_depth_dif - fabsf(d - dcen));
-                    wd += w;
-                    sumd += w * d;
-                }
+        for (int i = 0; i < (kPatch + kBlock - 1) / kBlock; i++) {
+            const int e = (int)threadIdx.x + i * kBlock;
+            const int vv = r0 + e % kPR, uu = c0 + e / kPR;
+            if (e < kPatch && vv >= 0 && vv < pr && uu >= 0 && uu < pc) {
+                const unsigned ip = (unsigned)vv + (unsigned)uu * (unsigned)pr;
+                sCol[e] = gptr(p.colp)[ip];
+                sDep[e] = gptr(p.depp)[ip];
             }
+        }
+        __syncthreads();
+        if (!on) return;
+        // .cu:357-424, taps in the reference's (k outer, l inner) order
+        const int c = (2 * (v - v0) + 2) + (2 * (u - u0) + 2) * kPR;   // (2v, 2u) in the patch
+        const float dcen = sDep[c];
+        float sumd = 0.f, sumc = 0.f, wd = 0.f, wc = 0.f;
+        const bool inner = (v > 0) && (v < (int)rows_i - 1) && (u > 0) && (u < (int)cols_i - 1);
+        if (inner) {
+            // every tap is in range.  A rejected depth tap adds +0 to wd and sumd instead of
+            // being skipped: both sums start at +0 and only add positive terms, so they are
+            // never -0 and adding +0 leaves them unchanged.
+#pragma unroll
+            for (int k = -2; k < 3; k++)
+#pragma unroll
+                for (int l = -2; l < 3; l++) {
+                    const float g = p.g[12 + k + 5 * l];
+                    const int s = c + k + l * kPR;
+                    sumc += g * sCol[s];
+                    const float d = sDep[s];
+                    const bool take = (d > 0.f) && (fabsf(d - dcen) < max_depth_dif);
+                    const float w = g * (max_depth_dif - fabsf(d - dcen));
+                    wd += take ? w : 0.f;
+                    sumd += take ? w * d : 0.f;
+                }
+            gptr(p.col)[index] = sumc;
+        } else {
+            for (int k = -2; k < 3; k++)
+                for (int l = -2; l < 3; l++) {
+                    const int vv = 2 * v + k, uu = 2 * u + l;
+                    if (!(vv >= 0 && vv < pr && uu >= 0 && uu < pc)) continue;
+                    const float g = p.g[12 + k + 5 * l];
+                    const int s = c + k + l * kPR;
+                    sumc += g * sCol[s];
+                    wc += g;
+                    const float d = sDep[s];
+                    if ((d > 0.f) && (fabsf(d - dcen) < max_depth_dif)) {
+                        const float w = g * (max_depth_dif - fabsf(d - dcen));
+                        wd += w;
+                        sumd += w * d;
+                    }
+                }
+            gptr(p.col)[index] = sumc / wc;
+        }
         depth = (sumd > 0.f) ? sumd / wd : 0.f;
-        p.dep[index] = depth;
-        p.col[index] = inner ? sumc : sumc / wc;
+        gptr(p.dep)[index] = depth;
     }
     const float disp_u = 0.5f * (float)(cols_i - 1);
     const float disp_v = 0.5f * (float)(rows_i - 1);
-    p.xx[index] = ((float)u - disp_u) * depth * p.inv_f_i;
-    p.yy[index] = ((float)v - disp_v) * depth * p.inv_f_i;
+    gptr(p.xx)[index] = ((float)u - disp_u) * depth * p.inv_f_i;
+    gptr(p.yy)[index] = ((float)v - disp_v) * depth * p.inv_f_i;
 }
 
 // ------------------------------------------------------------------ entry points
@@ -485,9 +531,9 @@ hipError_t launch_filter_motion(const LevelArgs &a, hipStream_t s)
 
 hipError_t launch_pyramid2(const Pyr2Args &a, hipStream_t s)
 {
-    const unsigned n = a.frame[0].rows_i * a.frame[0].cols_i;
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pyramid2, dim3((n + kBlock - 1) / kBlock, 2), dim3(kBlock), 0, s, a);
+    const unsigned rows = a.frame[0].rows_i, cols = a.frame[0].cols_i;
+    if (rows == 0 || cols == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pyramid2, dim3(tiles_of(rows, cols), 2), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 
@@ -516,7 +562,8 @@ unsigned batch_blocks_per_level(unsigned pixels) { return (pixels + kBlock - 1) 
 
 unsigned batch_blocks(BatchKernel k, unsigned rows, unsigned cols)
 {
-    if (k == BatchKernel::LevelInit || k == BatchKernel::FilterMotion) return tiles_of(rows, cols);
+    if (k == BatchKernel::LevelInit || k == BatchKernel::FilterMotion || k == BatchKernel::Pyramid)
+        return tiles_of(rows, cols);
     return batch_blocks_per_level(rows * cols);
 }
 

@@ -2,7 +2,7 @@
 projection's square root and reciprocal, and the step sizes' divisions of 0.5 and 1).
 
 The iteration kernels' unit-disc projection (pdflow_cudalib.cu:849-859) takes 1/sqrt(x) on
-x > 1. It uses a square root without the range handling that the compiler wraps around
+x >= 1. It uses a square root without the range handling that the compiler wraps around
 v_sqrt_f32, and a reciprocal that is the v_rcp_f32 estimate after one Newton step, without the
 compiler's residual corrections and range handling. Every result must equal sqrtf and "/" bit for
 bit. These tests evaluate every float of each range (about 10^9 inputs apiece) on the GPU and
@@ -13,7 +13,6 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 ONE = 0x3F800000            # 1.0f
-NEXT_AFTER_ONE = 0x3F800001
 INF = 0x7F800000
 TWO_TO_64 = 0x5F800000      # 2^64
 TWO_TO_M94 = 0x10800000     # 2^-94
@@ -27,9 +26,9 @@ def lib(gpu_available):
 
 
 @pytest.mark.parametrize("which,lo,hi,what", [
-    (0, NEXT_AFTER_ONE, INF, "sqrt on (1, inf]"),
+    (0, ONE, INF, "sqrt on [1, inf]"),
     (1, ONE, TWO_TO_64 - 1, "reciprocal on [1, 2^64)"),
-    (2, NEXT_AFTER_ONE, INF, "1/sqrt on (1, inf]"),
+    (2, ONE, INF, "1/sqrt on [1, inf]"),
     (3, TWO_TO_M94, TWO_TO_124, "1/d on [2^-94, 2^124]"),
     (4, TWO_TO_M94, TWO_TO_124, "0.5/d on [2^-94, 2^124]"),
 ])
