@@ -224,24 +224,46 @@ __device__ __forceinline__ void sort9(float (&k)[9], float (&w)[9])
     }
 }
 
+// sort9 of three (key, weight) arrays in lockstep: the same network, each compare-exchange
+// done for all three in turn so that their independent instructions interleave
+__device__ __forceinline__ void sort9x3(float (&ka)[9], float (&wa)[9], float (&kb)[9],
+                                        float (&wb)[9], float (&kc)[9], float (&wc)[9])
+{
+    auto ce = [](float (&k)[9], float (&w)[9], int i) {
+        const bool swap = k[i - 1] > k[i];
+        const float lo = swap ? k[i] : k[i - 1], hi = swap ? k[i - 1] : k[i];
+        const float wl = swap ? w[i] : w[i - 1], wh = swap ? w[i - 1] : w[i];
+        k[i - 1] = lo; k[i] = hi; w[i - 1] = wl; w[i] = wh;
+    };
+#pragma unroll
+    for (int round = 0; round < 9; round++) {
+#pragma unroll
+        for (int i = 1 + (round & 1); i < 9; i += 2) {
+            ce(ka, wa, i);
+            ce(kb, wb, i);
+            ce(kc, wc, i);
+        }
+    }
+}
+
 __device__ __forceinline__ float wmedian_pick(const float (&fld)[9], const float (&cum)[9], float med)
 {
-    // .cu:1103-1110 with register-only indexing
-    unsigned cont = 0;
-    bool go = true;
+    // .cu:1103-1110: cont = the first index with med <= cum[cont] (at most 8 here).  The
+    // values at cont - 1 and cont are carried along while med > cum[i] holds, so no index is
+    // formed; cont = 0 returns fld[0].
+    bool go = med > cum[0];
+    const bool interp = go;
+    float cl = cum[0], fl = fld[0], cr = cum[1], fr = fld[1];
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
+    for (int i = 1; i < 8; i++) {
         go = go && (med > cum[i]);
-        cont += go ? 1u : 0u;
+        cl = go ? cum[i] : cl;
+        fl = go ? fld[i] : fl;
+        cr = go ? cum[i + 1] : cr;
+        fr = go ? fld[i + 1] : fr;
     }
-    if (cont == 0) return fld[0];
-    float cr = 0.f, cl = 0.f, fr = 0.f, fl = 0.f;
-#pragma unroll
-    for (int i = 0; i < 9; i++) {
-        if ((unsigned)i == cont) { cr = cum[i]; fr = fld[i]; }
-        if ((unsigned)i + 1u == cont) { cl = cum[i]; fl = fld[i]; }
-    }
-    return ((cr - med) * fl + (med - cl) * fr) / (cr - cl);
+    const float r = ((cr - med) * fl + (med - cl) * fr) / (cr - cl);
+    return interp ? r : fld[0];
 }
 
 }  // namespace pdf

@@ -292,34 +292,52 @@ __device__ __forceinline__ void filter_motion_body(const LevelArgs &a, unsigned 
     float ou, ov, ow;
     if (d0 > 0.f) {
         // .cu:1057-1132: taps in the reference's order (k = v offset outer, l = u offset inner)
+        // the weight 1 / (1 + 5 dz^2 + 10 dt^2) has a denominator >= 1 (or NaN), which the short
+        // exact reciprocal covers up to 2^124 and the full division beyond
+        auto weight = [&](int h) {
+            const float dz = d0 - sZ[h], dt = sT[h];
+            return div_half_or_one(1.f, 1.f + 5.f * (dz * dz) + 10.f * (dt * dt));
+        };
         float ku[9], kv[9], kw[9], wu[9], wv[9], ww[9];
-        int n = 0;
+        const bool interior = v > 0 && v < (int)rows_i - 1 && u > 0 && u < (int)cols_i - 1;
+        if (interior) {
+            // all nine taps exist: tap s goes to slot s
 #pragma unroll
-        for (int s = 0; s < 9; s++) {
-            ku[s] = kv[s] = kw[s] = __builtin_inff();
-            wu[s] = wv[s] = ww[s] = 0.f;
-        }
+            for (int k = -1; k < 2; k++)
 #pragma unroll
-        for (int k = -1; k < 2; k++)
+                for (int l = -1; l < 2; l++) {
+                    const int s = (k + 1) * 3 + (l + 1), h = hc + k + l * HR;
+                    ku[s] = sU[h]; kv[s] = sV[h]; kw[s] = sW[h];
+                    wu[s] = wv[s] = ww[s] = weight(h);
+                }
+        } else {
+            // the existing taps fill the first slots in order; the others carry key +inf and
+            // weight 0
+            int n = 0;
 #pragma unroll
-            for (int l = -1; l < 2; l++) {
-                const int r = v + k, c = u + l;
-                const bool ok = !(r < 0 || r >= (int)rows_i || c < 0 || c >= (int)cols_i);
-                const int h = ok ? hc + k + l * HR : hc;
-                const float dz = d0 - sZ[h], dt = sT[h];
-                const float pres = 1.f / (1.f + 5.f * (dz * dz) + 10.f * (dt * dt));
-                const float fu = sU[h], fv = sV[h], fw = sW[h];
-#pragma unroll
-                for (int s = 0; s < 9; s++)
-                    if (ok && s == n) {
-                        ku[s] = fu; kv[s] = fv; kw[s] = fw;
-                        wu[s] = pres; wv[s] = pres; ww[s] = pres;
-                    }
-                n += ok ? 1 : 0;
+            for (int s = 0; s < 9; s++) {
+                ku[s] = kv[s] = kw[s] = __builtin_inff();
+                wu[s] = wv[s] = ww[s] = 0.f;
             }
-    sort9(ku, wu);
-        sort9(kv, wv);
-        sort9(kw, ww);
+#pragma unroll
+            for (int k = -1; k < 2; k++)
+#pragma unroll
+                for (int l = -1; l < 2; l++) {
+                    const int r = v + k, c = u + l;
+                    const bool ok = !(r < 0 || r >= (int)rows_i || c < 0 || c >= (int)cols_i);
+                    const int h = ok ? hc + k + l * HR : hc;
+                    const float pres = weight(h);
+                    const float fu = sU[h], fv = sV[h], fw = sW[h];
+#pragma unroll
+                    for (int s = 0; s < 9; s++)
+                        if (ok && s == n) {
+                            ku[s] = fu; kv[s] = fv; kw[s] = fw;
+                            wu[s] = pres; wv[s] = pres; ww[s] = pres;
+                        }
+                    n += ok ? 1 : 0;
+                }
+        }
+        sort9x3(ku, wu, kv, wv, kw, ww);
         float cu[9], cv[9], cw[9];
         cu[0] = wu[0]; cv[0] = wv[0]; cw[0] = ww[0];
 #pragma unroll
@@ -370,79 +388,84 @@ __device__ __forceinline__ void pyramid_body(const PyrArgs &p, unsigned tile)
 {
     __shared__ float sCol[kPatch], sDep[kPatch];
     const unsigned rows_i = p.rows_i, cols_i = p.cols_i;
+    const float max_depth_dif = 0.1f;
+    const float disp_u = 0.5f * (float)(cols_i - 1);
+    const float disp_v = 0.5f * (float)(rows_i - 1);
+    if (p.level == 0) {
+        // a copy: pixels in column-major order (the tiles outnumber rows_i * cols_i / kBlock),
+        // so that a wave's accesses are 256 contiguous bytes
+        const unsigned index = tile * kBlock + threadIdx.x;
+        if (index >= rows_i * cols_i) return;
+        const unsigned v = index % rows_i, u = index / rows_i;
+        gptr(p.col)[index] = gptr(p.colour_wf)[index];
+        const float depth = gptr(p.depth_wf)[index];
+        gptr(p.dep)[index] = depth;
+        gptr(p.xx)[index] = ((float)u - disp_u) * depth * p.inv_f_i;
+        gptr(p.yy)[index] = ((float)v - disp_v) * depth * p.inv_f_i;
+        return;
+    }
     const unsigned tv = (rows_i + kTR - 1) / kTR;
     const int v0 = (int)(tile % tv) * kTR, u0 = (int)(tile / tv) * kTC;
     const int v = v0 + (int)threadIdx.x % kTR, u = u0 + (int)threadIdx.x / kTR;
     const bool on = v < (int)rows_i && u < (int)cols_i;
     const unsigned index = (unsigned)v + (unsigned)u * rows_i;   // read only when on
-    const float max_depth_dif = 0.1f;
-    float depth;
-    if (p.level == 0) {
-        if (!on) return;
-        gptr(p.col)[index] = gptr(p.colour_wf)[index];
-        depth = gptr(p.depth_wf)[index];
-        gptr(p.dep)[index] = depth;
-    } else {
-        const int pr = 2 * (int)rows_i, pc = 2 * (int)cols_i;   // the finer level
-        const int r0 = 2 * v0 - 2, c0 = 2 * u0 - 2;
+    const int pr = 2 * (int)rows_i, pc = 2 * (int)cols_i;   // the finer level
+    const int r0 = 2 * v0 - 2, c0 = 2 * u0 - 2;
 #pragma unroll
-        for (int i = 0; i < (kPatch + kBlock - 1) / kBlock; i++) {
-            const int e = (int)threadIdx.x + i * kBlock;
-            const int vv = r0 + e % kPR, uu = c0 + e / kPR;
-            if (e < kPatch && vv >= 0 && vv < pr && uu >= 0 && uu < pc) {
-                const unsigned ip = (unsigned)vv + (unsigned)uu * (unsigned)pr;
-                sCol[e] = gptr(p.colp)[ip];
-                sDep[e] = gptr(p.depp)[ip];
-            }
+    for (int i = 0; i < (kPatch + kBlock - 1) / kBlock; i++) {
+        const int e = (int)threadIdx.x + i * kBlock;
+        const int vv = r0 + e % kPR, uu = c0 + e / kPR;
+        if (e < kPatch && vv >= 0 && vv < pr && uu >= 0 && uu < pc) {
+            const unsigned ip = (unsigned)vv + (unsigned)uu * (unsigned)pr;
+            sCol[e] = gptr(p.colp)[ip];
+            sDep[e] = gptr(p.depp)[ip];
         }
-        __syncthreads();
-        if (!on) return;
-        // .cu:357-424, taps in the reference's (k outer, l inner) order
-        const int c = (2 * (v - v0) + 2) + (2 * (u - u0) + 2) * kPR;   // (2v, 2u) in the patch
-        const float dcen = sDep[c];
-        float sumd = 0.f, sumc = 0.f, wd = 0.f, wc = 0.f;
-        const bool inner = (v > 0) && (v < (int)rows_i - 1) && (u > 0) && (u < (int)cols_i - 1);
-        if (inner) {
-            // every tap is in range.  A rejected depth tap adds +0 to wd and sumd instead of
-            // being skipped: both sums start at +0 and only add positive terms, so they are
-            // never -0 and adding +0 leaves them unchanged.
-#pragma unroll
-            for (int k = -2; k < 3; k++)
-#pragma unroll
-                for (int l = -2; l < 3; l++) {
-                    const float g = p.g[12 + k + 5 * l];
-                    const int s = c + k + l * kPR;
-                    sumc += g * sCol[s];
-                    const float d = sDep[s];
-                    const bool take = (d > 0.f) && (fabsf(d - dcen) < max_depth_dif);
-                    const float w = g * (max_depth_dif - fabsf(d - dcen));
-                    wd += take ? w : 0.f;
-                    sumd += take ? w * d : 0.f;
-                }
-            gptr(p.col)[index] = sumc;
-        } else {
-            for (int k = -2; k < 3; k++)
-                for (int l = -2; l < 3; l++) {
-                    const int vv = 2 * v + k, uu = 2 * u + l;
-                    if (!(vv >= 0 && vv < pr && uu >= 0 && uu < pc)) continue;
-                    const float g = p.g[12 + k + 5 * l];
-                    const int s = c + k + l * kPR;
-                    sumc += g * sCol[s];
-                    wc += g;
-                    const float d = sDep[s];
-                    if ((d > 0.f) && (fabsf(d - dcen) < max_depth_dif)) {
-                        const float w = g * (max_depth_dif - fabsf(d - dcen));
-                        wd += w;
-                        sumd += w * d;
-                    }
-                }
-            gptr(p.col)[index] = sumc / wc;
-        }
-        depth = (sumd > 0.f) ? sumd / wd : 0.f;
-        gptr(p.dep)[index] = depth;
     }
-    const float disp_u = 0.5f * (float)(cols_i - 1);
-    const float disp_v = 0.5f * (float)(rows_i - 1);
+    __syncthreads();
+    if (!on) return;
+    // .cu:357-424, taps in the reference's (k outer, l inner) order
+    const int c = (2 * (v - v0) + 2) + (2 * (u - u0) + 2) * kPR;   // (2v, 2u) in the patch
+    const float dcen = sDep[c];
+    float sumd = 0.f, sumc = 0.f, wd = 0.f, wc = 0.f;
+    const bool inner = (v > 0) && (v < (int)rows_i - 1) && (u > 0) && (u < (int)cols_i - 1);
+    if (inner) {
+        // every tap is in range.  A rejected depth tap adds +0 to wd and sumd instead of
+        // being skipped: both sums start at +0 and only add positive terms, so they are
+        // never -0 and adding +0 leaves them unchanged.
+#pragma unroll
+        for (int k = -2; k < 3; k++)
+#pragma unroll
+            for (int l = -2; l < 3; l++) {
+                const float g = p.g[12 + k + 5 * l];
+                const int s = c + k + l * kPR;
+                sumc += g * sCol[s];
+                const float d = sDep[s];
+                const bool take = (d > 0.f) && (fabsf(d - dcen) < max_depth_dif);
+                const float w = g * (max_depth_dif - fabsf(d - dcen));
+                wd += take ? w : 0.f;
+                sumd += take ? w * d : 0.f;
+            }
+        gptr(p.col)[index] = sumc;
+    } else {
+        for (int k = -2; k < 3; k++)
+            for (int l = -2; l < 3; l++) {
+                const int vv = 2 * v + k, uu = 2 * u + l;
+                if (!(vv >= 0 && vv < pr && uu >= 0 && uu < pc)) continue;
+                const float g = p.g[12 + k + 5 * l];
+                const int s = c + k + l * kPR;
+                sumc += g * sCol[s];
+                wc += g;
+                const float d = sDep[s];
+                if ((d > 0.f) && (fabsf(d - dcen) < max_depth_dif)) {
+                    const float w = g * (max_depth_dif - fabsf(d - dcen));
+                    wd += w;
+                    sumd += w * d;
+                }
+            }
+        gptr(p.col)[index] = sumc / wc;
+    }
+    const float depth = (sumd > 0.f) ? sumd / wd : 0.f;
+    gptr(p.dep)[index] = depth;
     gptr(p.xx)[index] = ((float)u - disp_u) * depth * p.inv_f_i;
     gptr(p.yy)[index] = ((float)v - disp_v) * depth * p.inv_f_i;
 }
