@@ -208,7 +208,10 @@ int pdf_batch_set_tb_shape(pdf_batch *b, int shape);
  * composition: a hit applies the cached choice without timing, a miss appends the new line.  The
  * batch's inputs must be set (it is solved several times). */
 int pdf_batch_autotune(pdf_batch *b, const char *cache_path, int *shape, int *k, float *ms);
-/* device pointers of pair's four input planes (the caller may fill them directly) */
+/* device pointers of pair's four input planes (the caller may fill them directly).  They are
+ * also level 0 of the pair's colour and depth pyramids, so new inputs take effect there at once:
+ * a continuation or re-solve after new inputs would mix them with the pyramid levels the last
+ * solve built.  Set new inputs, then solve. */
 int pdf_batch_inputs_device(pdf_batch *b, unsigned pair, float **I1, float **Z1, float **I2,
                             float **Z2);
 /* Warm start of one pair (as pdf_solve_warm): the pair's solve starts at start_level (0 =
@@ -219,7 +222,8 @@ int pdf_batch_set_start_level(pdf_batch *b, unsigned pair, unsigned start_level,
 /* per-pair iterations per level, coarse first, at most 100000 each (invalidates the plan when
  * they change) */
 int pdf_batch_set_schedule(pdf_batch *b, unsigned pair, const unsigned *num_max_iter);
-/* blocking upload of one pair's frames from host planes */
+/* blocking upload of one pair's frames from host planes (into the planes of
+ * pdf_batch_inputs_device) */
 int pdf_batch_set_inputs(pdf_batch *b, unsigned pair, const float *I1, const float *Z1,
                          const float *I2, const float *Z2);
 /* solves every pair (two pyramids + coarse-to-fine solve each); asynchronous on the batch's

@@ -641,7 +641,9 @@ int pdf_batch_create(pdf_batch **out, unsigned npairs, const unsigned *rows, con
         }
         const size_t P0 = (size_t)p.width * p.height;
         bytes += 4 * rnd(P0);
-        for (unsigned l = 0; l < p.pyr_levels; l++) bytes += PDF_NUM_PYR_KINDS * rnd(P0 >> (2 * l));
+        // level 0 of the colour and depth pyramids is the input planes themselves
+        bytes += (PDF_NUM_PYR_KINDS - 4) * rnd(P0);
+        for (unsigned l = 1; l < p.pyr_levels; l++) bytes += PDF_NUM_PYR_KINDS * rnd(P0 >> (2 * l));
         bytes += (sizeof(kUsed) / sizeof(kUsed[0]) + pdf::S_NUM) * rnd(P0);
         bytes += 3 * rnd(P0);                                   // warm-start planes
     }
@@ -657,8 +659,15 @@ int pdf_batch_create(pdf_batch **out, unsigned npairs, const unsigned *rows, con
     for (Pair &p : b->pairs) {
         const size_t P0 = (size_t)p.width * p.height;
         for (int k = 0; k < 4; k++) p.input[k] = take(P0);
+        p.pyr[PDF_PYR_COLOUR_OLD][0] = p.input[0];
+        p.pyr[PDF_PYR_DEPTH_OLD][0] = p.input[1];
+        p.pyr[PDF_PYR_COLOUR][0] = p.input[2];
+        p.pyr[PDF_PYR_DEPTH][0] = p.input[3];
         for (unsigned l = 0; l < p.pyr_levels; l++)
-            for (int k = 0; k < PDF_NUM_PYR_KINDS; k++) p.pyr[k][l] = take(P0 >> (2 * l));
+            for (int k = 0; k < PDF_NUM_PYR_KINDS; k++)
+                if (l > 0 || (k != PDF_PYR_COLOUR_OLD && k != PDF_PYR_DEPTH_OLD && k != PDF_PYR_COLOUR &&
+                              k != PDF_PYR_DEPTH))
+                    p.pyr[k][l] = take(P0 >> (2 * l));
         for (int id : kUsed) p.f[id] = take(P0);
         for (int k = 0; k < pdf::S_NUM; k++) p.alt[kState[k]] = take(P0);
         p.warm = take(3 * P0);

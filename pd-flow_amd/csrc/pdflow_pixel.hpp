@@ -175,35 +175,47 @@ __device__ __forceinline__ Tap make_tap(unsigned rows_i, unsigned cols_i, float 
     return t;
 }
 
-// mat is a plane in global memory
-__device__ __forceinline__ float bilinear(const float *plane, unsigned rows_i, const Tap &t)
+// The four neighbours (fv, fu), (sv, fu), (fv, su), (sv, su) of a clamped tap in a plane in global
+// memory, loaded together.  Every case of interpolatePixel and interpolatePixelDepth reads only
+// these: at an integer coordinate the rounded coordinate is that integer, and otherwise rintf
+// rounds it to fu or su (fv or sv).
+struct Corners {
+    float ff, sf, fs, ss;   // (v, u) = (fv, fu), (sv, fu), (fv, su), (sv, su)
+};
+
+__device__ __forceinline__ Corners corners(const float *plane, unsigned rows_i, const Tap &t)
+{
+    const auto mat = gptr(plane);
+    return {mat[t.fv + rows_i * t.fu], mat[t.sv + rows_i * t.fu], mat[t.fv + rows_i * t.su],
+            mat[t.sv + rows_i * t.su]};
+}
+
+__device__ __forceinline__ float bilinear(const Corners &c, const Tap &t)
 {
     // interpolatePixel (.cu:666-681) after clamping
-    const auto mat = gptr(plane);
-    if (t.su == t.fu && t.sv == t.fv) return mat[(unsigned)rintf(t.iv + (float)rows_i * t.iu)];
-    if (t.su == t.fu) {
-        const unsigned cu = (unsigned)rintf(t.iu);
-        return ((float)t.sv - t.iv) * mat[t.fv + rows_i * cu] + (t.iv - (float)t.fv) * mat[t.sv + rows_i * cu];
-    }
-    if (t.sv == t.fv) {
-        const unsigned cv = (unsigned)rintf(t.iv);
-        return ((float)t.su - t.iu) * mat[cv + rows_i * t.fu] + (t.iu - (float)t.fu) * mat[cv + rows_i * t.su];
-    }
-    const float top = ((float)t.su - t.iu) * mat[t.sv + rows_i * t.fu] + (t.iu - (float)t.fu) * mat[t.sv + rows_i * t.su];
-    const float bot = ((float)t.su - t.iu) * mat[t.fv + rows_i * t.fu] + (t.iu - (float)t.fu) * mat[t.fv + rows_i * t.su];
+    const bool eu = t.su == t.fu, ev = t.sv == t.fv;
+    if (eu && ev) return c.ff;
+    if (eu) return ((float)t.sv - t.iv) * c.ff + (t.iv - (float)t.fv) * c.sf;
+    if (ev) return ((float)t.su - t.iu) * c.ff + (t.iu - (float)t.fu) * c.fs;
+    const float top = ((float)t.su - t.iu) * c.sf + (t.iu - (float)t.fu) * c.ss;
+    const float bot = ((float)t.su - t.iu) * c.ff + (t.iu - (float)t.fu) * c.fs;
     return ((float)t.sv - t.iv) * bot + (t.iv - (float)t.fv) * top;
+}
+
+__device__ __forceinline__ float bilinear(const float *plane, unsigned rows_i, const Tap &t)
+{
+    return bilinear(corners(plane, rows_i, t), t);
 }
 
 __device__ __forceinline__ float bilinear_depth(const float *plane, unsigned rows_i, const Tap &t)
 {
     // interpolatePixelDepth (.cu:696-720): nearest neighbour (half to even) if a tap is 0
-    const auto mat = gptr(plane);
-    if (mat[t.sv + rows_i * t.su] == 0.f || mat[t.sv + rows_i * t.fu] == 0.f ||
-        mat[t.fv + rows_i * t.su] == 0.f || mat[t.fv + rows_i * t.fu] == 0.f) {
+    const Corners c = corners(plane, rows_i, t);
+    if (c.ss == 0.f || c.sf == 0.f || c.fs == 0.f || c.ff == 0.f) {
         const unsigned ru = (unsigned)rintf(t.iu), rv = (unsigned)rintf(t.iv);
-        return mat[rv + rows_i * ru];
+        return ru == t.fu ? (rv == t.fv ? c.ff : c.sf) : (rv == t.fv ? c.fs : c.ss);
     }
-    return bilinear(plane, rows_i, t);
+    return bilinear(c, t);
 }
 
 // Stable sort of 9 (key, weight) pairs by key.  The reference uses a bubble sort with a

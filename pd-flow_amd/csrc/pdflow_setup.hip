@@ -148,52 +148,56 @@ __device__ __forceinline__ void rij_gradients_body(const LevelArgs &a, unsigned 
     const unsigned rows_i = a.rows, cols_i = a.cols, r = rows_i;
     if (index >= rows_i * cols_i) return;
     const unsigned v = index % rows_i, u = index / rows_i;
-    // K5 (.cu:725-779)
-    float dxu = 0.f, dzu = 0.f, dxu2 = 0.f, dzu2 = 0.f;
-    float dyv = 0.f, dzv = 0.f, dyv2 = 0.f, dzv2 = 0.f;
-    if (u != cols_i - 1) {
-        dxu = gptr(a.Xold)[index + r] - gptr(a.Xold)[index];
-        dzu = gptr(a.Zold)[index + r] - gptr(a.Zold)[index];
-        dxu2 = gptr(a.X)[index + r] - gptr(a.X)[index];
-        dzu2 = gptr(a.Z)[index + r] - gptr(a.Z)[index];
-    }
-    if (v != rows_i - 1) {
-        dyv = gptr(a.Yold)[index + 1] - gptr(a.Yold)[index];
-        dzv = gptr(a.Zold)[index + 1] - gptr(a.Zold)[index];
-        dyv2 = gptr(a.Y)[index + 1] - gptr(a.Y)[index];
-        dzv2 = gptr(a.Z)[index + 1] - gptr(a.Z)[index];
-    }
+    const bool uf = u == 0, ul = u == cols_i - 1, vf = v == 0, vl = v == rows_i - 1;
+    // Every value is loaded up front, so the loads are in flight together.  A neighbour index
+    // that would leave the plane is clamped to the pixel itself: the load stays in bounds, and
+    // the border cases below never use that value.
+    const unsigned iu1 = ul ? index : index + r, iu0 = uf ? index : index - r;
+    const unsigned iv1 = vl ? index : index + 1, iv0 = vf ? index : index - 1;
+    const auto X = gptr(a.X), Y = gptr(a.Y), Z = gptr(a.Z), I = gptr(a.I);
+    const auto Xo = gptr(a.Xold), Yo = gptr(a.Yold), Zo = gptr(a.Zold);
+    const float x_c = X[index], x_u1 = X[iu1], x_u0 = X[iu0];
+    const float y_c = Y[index], y_v1 = Y[iv1], y_v0 = Y[iv0];
+    const float z_c = Z[index], z_u1 = Z[iu1], z_u0 = Z[iu0], z_v1 = Z[iv1], z_v0 = Z[iv0];
+    const float i_c = I[index], i_u1 = I[iu1], i_u0 = I[iu0], i_v1 = I[iv1], i_v0 = I[iv0];
+    const float xo_c = Xo[index], xo_u1 = Xo[iu1], yo_c = Yo[index], yo_v1 = Yo[iv1];
+    const float zo_c = Zo[index], zo_u1 = Zo[iu1], zo_v1 = Zo[iv1];
+    // K5 (.cu:725-779): the differences towards u+1 and v+1 are 0 in the last column / row
+    const float dxu = ul ? 0.f : xo_u1 - xo_c, dzu = ul ? 0.f : zo_u1 - zo_c;
+    const float dxu2 = ul ? 0.f : x_u1 - x_c, dzu2 = ul ? 0.f : z_u1 - z_c;
+    const float dyv = vl ? 0.f : yo_v1 - yo_c, dzv = vl ? 0.f : zo_v1 - zo_c;
+    const float dyv2 = vl ? 0.f : y_v1 - y_c, dzv2 = vl ? 0.f : z_v1 - z_c;
     const float ri2 = rweight(dxu2, dzu2), rj2 = rweight(dyv2, dzv2);
-    gptr(a.f[PDF_RI])[index] = rweight(dxu, dzu);
-    gptr(a.f[PDF_RJ])[index] = rweight(dyv, dzv);
-    gptr(a.f[PDF_RI_2])[index] = ri2;
-    gptr(a.f[PDF_RJ_2])[index] = rj2;
+    const float ri = rweight(dxu, dzu), rj = rweight(dyv, dzv);
     // K6 (.cu:574-627); ri_2 at (v, u-1) and rj_2 at (v-1, u) recomputed (u-1 and v-1 are
     // never the last column/row there, so their differences always exist)
-    const auto I = gptr(a.I), Z = gptr(a.Z);
     float dcu, ddu, dcv, ddv;
-    if (u == 0) {
-        dcu = I[index + r] - I[index];
-        ddu = Z[index + r] - Z[index];
-    } else if (u == cols_i - 1) {
-        dcu = I[index] - I[index - r];
-        ddu = Z[index] - Z[index - r];
+    if (uf) {
+        dcu = i_u1 - i_c;
+        ddu = z_u1 - z_c;
+    } else if (ul) {
+        dcu = i_c - i_u0;
+        ddu = z_c - z_u0;
     } else {
-        const float rp = rweight(gptr(a.X)[index] - gptr(a.X)[index - r], Z[index] - Z[index - r]);
-        dcu = wdiff(ri2, rp, I[index + r], I[index], I[index - r]);
-        ddu = (Z[index] > 0.f) ? wdiff(ri2, rp, Z[index + r], Z[index], Z[index - r]) : 0.f;
+        const float rp = rweight(x_c - x_u0, z_c - z_u0);
+        dcu = wdiff(ri2, rp, i_u1, i_c, i_u0);
+        ddu = (z_c > 0.f) ? wdiff(ri2, rp, z_u1, z_c, z_u0) : 0.f;
     }
-    if (v == 0) {
-        dcv = I[index + 1] - I[index];
-        ddv = Z[index + 1] - Z[index];
-    } else if (v == rows_i - 1) {
-        dcv = I[index] - I[index - 1];
-        ddv = Z[index] - Z[index - 1];
+    if (vf) {
+        dcv = i_v1 - i_c;
+        ddv = z_v1 - z_c;
+    } else if (vl) {
+        dcv = i_c - i_v0;
+        ddv = z_c - z_v0;
     } else {
-        const float rp = rweight(gptr(a.Y)[index] - gptr(a.Y)[index - 1], Z[index] - Z[index - 1]);
-        dcv = wdiff(rj2, rp, I[index + 1], I[index], I[index - 1]);
-        ddv = (Z[index] > 0.f) ? wdiff(rj2, rp, Z[index + 1], Z[index], Z[index - 1]) : 0.f;
+        const float rp = rweight(y_c - y_v0, z_c - z_v0);
+        dcv = wdiff(rj2, rp, i_v1, i_c, i_v0);
+        ddv = (z_c > 0.f) ? wdiff(rj2, rp, z_v1, z_c, z_v0) : 0.f;
     }
+    gptr(a.f[PDF_RI])[index] = ri;
+    gptr(a.f[PDF_RJ])[index] = rj;
+    gptr(a.f[PDF_RI_2])[index] = ri2;
+    gptr(a.f[PDF_RJ_2])[index] = rj2;
     gptr(a.f[PDF_DCU_AUX])[index] = dcu;
     gptr(a.f[PDF_DDU_AUX])[index] = ddu;
     gptr(a.f[PDF_DCV_AUX])[index] = dcv;
@@ -393,13 +397,14 @@ __device__ __forceinline__ void pyramid_body(const PyrArgs &p, unsigned tile)
     const float disp_v = 0.5f * (float)(rows_i - 1);
     if (p.level == 0) {
         // a copy: pixels in column-major order (the tiles outnumber rows_i * cols_i / kBlock),
-        // so that a wave's accesses are 256 contiguous bytes
+        // so that a wave's accesses are 256 contiguous bytes.  A batch's level-0 colour and
+        // depth planes are its input planes themselves, and only xx, yy are written.
         const unsigned index = tile * kBlock + threadIdx.x;
         if (index >= rows_i * cols_i) return;
         const unsigned v = index % rows_i, u = index / rows_i;
-        gptr(p.col)[index] = gptr(p.colour_wf)[index];
         const float depth = gptr(p.depth_wf)[index];
-        gptr(p.dep)[index] = depth;
+        if (p.col != p.colour_wf) gptr(p.col)[index] = gptr(p.colour_wf)[index];
+        if (p.dep != p.depth_wf) gptr(p.dep)[index] = depth;
         gptr(p.xx)[index] = ((float)u - disp_u) * depth * p.inv_f_i;
         gptr(p.yy)[index] = ((float)v - disp_v) * depth * p.inv_f_i;
         return;
