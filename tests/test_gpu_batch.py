@@ -46,6 +46,42 @@ def test_mixed_batch_matches_single_pairs(gpu_available, k, graph, shape):
     say(f"  {b.launch_count()} launches per batched solve")
 
 
+def test_batch_pyramids_match_the_oracle(gpu_available):
+    """Every pyramid plane a batched solve builds (both frames; colour, depth, xx, yy; every
+    level; pairs of four sizes) equals the oracle's bit for bit, and level 0 of a pair's colour
+    and depth pyramids is its input planes themselves."""
+    from oracle import Oracle
+    from pdflow import lib as plib
+    rows = [480, 240, 120, 60]
+    say(f"batched pyramids of {rows} against the oracle")
+    b = Batch(rows, iters=[[1] * 6, [1] * 5, [1] * 4, [1] * 3])
+    oracles = []
+    for i, r in enumerate(rows):
+        cam = 480 // r
+        I1, Z1, I2, Z2 = synth_pair_float(900 + i, 480 // cam, 640 // cam)
+        b.set_inputs(i, I1, Z1, I2, Z2)
+        o = Oracle(r, cam, threads=8)
+        o.copy_new_frames(I1, Z1)
+        o.gaussian_pyramid()
+        o.copy_new_frames(I2, Z2)
+        o.gaussian_pyramid()
+        oracles.append(o)
+    b.solve(graph=True)
+    b.synchronize()
+    for i, o in enumerate(oracles):
+        I1d, Z1d, I2d, Z2d = b.inputs_device(i)
+        for kind, addr in (("colour_old", I1d), ("depth_old", Z1d), ("colour", I2d), ("depth", Z2d)):
+            ptr = plib.ctypes.c_void_p()
+            r_, c_ = plib.ctypes.c_uint(), plib.ctypes.c_uint()
+            assert b.lib.pdf_batch_pyramid_device(b.ctx, i, plib.PYR_KINDS[kind], 0, plib.ctypes.byref(ptr),
+                                                  plib.ctypes.byref(r_), plib.ctypes.byref(c_)) == 0
+            assert ptr.value == addr, f"pair {i}: level-0 {kind} is not the input plane"
+        for lvl in range(o.pyr_levels):
+            for kind in plib.PYR_KINDS:
+                got = b.pyramid(i, kind, lvl)[0]
+                np.testing.assert_array_equal(got, o.get_pyramid(kind, lvl), err_msg=f"pair {i} {kind}[{lvl}]")
+
+
 def test_continue_equals_a_longer_finest_level(gpu_available):
     """pdf_batch_continue: n more finest-level iterations from the solved state give exactly
     the flow of a solve whose finest level ran n more; continuing twice adds up; the next
