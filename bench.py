@@ -34,7 +34,7 @@ BYTES_PER_PIXEL_ITER = 148     # 11 constant planes read + 13 state planes read 
 # of this workload, scaled by the counters' readings for a plain copy of known size (fetch x 2.0,
 # write x 1.0); the passes and the per-level figures are in
 # profiles/pd_iteration_batched_learned_vga256_hbm_traffic.txt
-TRAFFIC_TB_LEARNED_VGA256 = {2.0: 4.789e9, 2.5: 4.661e9, 4.0: 4.885e9}
+TRAFFIC_TB_LEARNED_VGA256 = {2.0: 4.789e9, 2.5: 4.660e9, 4.0: 4.885e9}
 
 
 def log(msg):
