@@ -673,6 +673,12 @@ int pdf_batch_create(pdf_batch **out, unsigned npairs, const unsigned *rows, con
         p.warm = take(3 * P0);
         p.start_level = 0;
     }
+    if ((size_t)(cur - b->arena) > bytes) {   // the sizing above and this carving must agree
+        g_batch_create_error = "internal error: batch arena carved beyond its size";
+        (void)hipFree(b->arena);
+        delete b;
+        return PDF_ERR_HIP;
+    }
     e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemsetAsync(b->arena, 0, bytes, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
