@@ -239,3 +239,48 @@ def test_trained_network_beats_the_ablation(pdflow):
     assert e_learned <= e_plain + 0.002, (e_learned, e_plain)
     # closer to the plain solution too
     assert learned.epe(plain).mean() < ablation.epe(plain).mean() + 0.005
+
+
+def test_two_learned_batches_in_flight_match_one_at_a_time(pdflow):
+    """The benchmark's pipeline: two learned batches, each with its own network instance and
+    stream, one solving while the other continues some pairs and re-solves another. Every flow
+    equals that of the same batch run alone."""
+    import os
+    from pdflow.train import cut_schedule
+    weights = os.path.join(os.path.dirname(pdflow.__file__), "weights", "warmstart_qvga.bin")
+    n, rows = 8, 240
+    cut = [cut_schedule(rows, 2.5)] * n
+    inputs = {s: pdflow.synth_pairs_float(s, n, rows, 320) for s in (9400, 9500)}
+
+    def make(seed):
+        net = pdflow.Net(max_batch=n, h=60, w=80)
+        net.load(weights)
+        b = pdflow.Batch([rows] * n, iters=cut)
+        for i in range(n):
+            b.set_inputs(i, *(p[i] for p in inputs[seed][:4]))
+        b.set_learned(net)
+        return b, net
+
+    def finish(b):
+        b.continue_finest([0, 3, 5], 16)
+        b.resolve_plain([1])
+
+    want = {}
+    for seed in (9400, 9500):
+        b, _ = make(seed)
+        b.solve(graph=True)
+        finish(b)
+        b.synchronize()
+        want[seed] = _flows(b, n)
+    (a, net_a), (c, net_c) = make(9400), make(9500)
+    for rep in range(2):
+        a.solve(graph=True)
+        c.solve(graph=True)          # in flight while a finishes
+        finish(a)
+        finish(c)
+        a.synchronize()
+        c.synchronize()
+        for seed, b in ((9400, a), (9500, c)):
+            for i, (got, exp) in enumerate(zip(_flows(b, n), want[seed])):
+                for k in range(3):
+                    assert np.array_equal(got[k], exp[k]), (rep, seed, i, k)
