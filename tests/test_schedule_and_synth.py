@@ -107,3 +107,17 @@ def test_oracle_regression_fixture():
     got = o.solve_pair(d["I1"], d["Z1"], d["I2"], d["Z2"])
     for name, g in zip(("dx", "dy", "dz"), got):
         np.testing.assert_array_equal(g, d[name])
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asking for more exits non-zero with a message, before any GPU
+    work."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU only" in (r.stdout + r.stderr)
+    assert '"metric"' not in r.stdout
