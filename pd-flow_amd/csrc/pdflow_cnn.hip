@@ -360,6 +360,20 @@ __global__ void k_upsample(const uint16_t *x, uint16_t *y, int nb, int h, int w,
     y[q * ys + off + ch] = x[((size_t)(n * h + Y / 2) * w + X / 2) * c + ch];
 }
 
+// k_upsample with 8 channels (16 bytes) per thread: c, ys and off multiples of 8, both tensors
+// 16-byte aligned (c8 = c / 8)
+__global__ void k_upsample8(const uint16_t *x, uint16_t *y, int nb, int h, int w, int c8, int ys, int off)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int H = 2 * h, W = 2 * w;
+    if (i >= (size_t)nb * H * W * c8) return;
+    const int ch = 8 * (int)(i % c8);
+    const size_t q = i / c8;
+    const int X = q % W, Y = (q / W) % H, n = q / (W * H);
+    const uint4 v = *reinterpret_cast<const uint4 *>(x + ((size_t)(n * h + Y / 2) * w + X / 2) * (8 * c8) + ch);
+    *reinterpret_cast<uint4 *>(y + q * ys + off + ch) = v;
+}
+
 // gradient of the upsample: dx[y][x] = sum of the four dy children (channel slice of dy)
 __global__ void k_upsample_grad(const uint16_t *dy, uint16_t *dx, int nb, int h, int w, int c, int ys, int off)
 {
@@ -389,6 +403,18 @@ __global__ void k_slice(const uint16_t *src, int ss, int soff, uint16_t *dst, in
     const uint16_t v = src[p * ss + soff + ch];
     uint16_t *d = dst + p * ds + doff + ch;
     *d = accumulate ? f2bf(bf2f(*d) + bf2f(v)) : v;
+}
+
+// k_slice without accumulation, 8 channels (16 bytes) per thread: c, strides and offsets
+// multiples of 8, both tensors 16-byte aligned (c8 = c / 8)
+__global__ void k_slice8(const uint16_t *src, int ss, int soff, uint16_t *dst, int ds, int doff, size_t npix, int c8)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= npix * c8) return;
+    const size_t p = i / c8;
+    const int ch = 8 * (int)(i % c8);
+    const uint4 v = *reinterpret_cast<const uint4 *>(src + p * ss + soff + ch);
+    *reinterpret_cast<uint4 *>(dst + p * ds + doff + ch) = v;
 }
 
 __global__ void k_f32_to_bf16(const float *x, uint16_t *y, size_t n)
@@ -536,9 +562,20 @@ hipError_t pool_grad(const uint16_t *dy, uint16_t *dx, int nb, int h, int w, int
     return hipGetLastError();
 }
 
+// 16-byte accesses are possible when every channel run starts on a 16-byte boundary
+static bool by8(const void *a, const void *b, int c, int s1, int o1, int s2, int o2)
+{
+    return c % 8 == 0 && s1 % 8 == 0 && o1 % 8 == 0 && s2 % 8 == 0 && o2 % 8 == 0 &&
+           reinterpret_cast<uintptr_t>(a) % 16 == 0 && reinterpret_cast<uintptr_t>(b) % 16 == 0;
+}
+
 hipError_t upsample(const uint16_t *x, uint16_t *y, int nb, int h, int w, int c, int ys, int off, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_upsample, g1((size_t)nb * 4 * h * w * c), dim3(256), 0, s, x, y, nb, h, w, c, ys, off);
+    if (by8(x, y, c, c, 0, ys, off))
+        hipLaunchKernelGGL(k_upsample8, g1((size_t)nb * 4 * h * w * (c / 8)), dim3(256), 0, s, x, y, nb, h, w, c / 8,
+                           ys, off);
+    else
+        hipLaunchKernelGGL(k_upsample, g1((size_t)nb * 4 * h * w * c), dim3(256), 0, s, x, y, nb, h, w, c, ys, off);
     return hipGetLastError();
 }
 
@@ -551,7 +588,10 @@ hipError_t upsample_grad(const uint16_t *dy, uint16_t *dx, int nb, int h, int w,
 hipError_t slice(const uint16_t *src, int ss, int soff, uint16_t *dst, int ds, int doff, size_t npix, int c, bool acc,
                  hipStream_t s)
 {
-    hipLaunchKernelGGL(k_slice, g1(npix * c), dim3(256), 0, s, src, ss, soff, dst, ds, doff, npix, c, acc ? 1 : 0);
+    if (!acc && by8(src, dst, c, ss, soff, ds, doff))
+        hipLaunchKernelGGL(k_slice8, g1(npix * (c / 8)), dim3(256), 0, s, src, ss, soff, dst, ds, doff, npix, c / 8);
+    else
+        hipLaunchKernelGGL(k_slice, g1(npix * c), dim3(256), 0, s, src, ss, soff, dst, ds, doff, npix, c, acc ? 1 : 0);
     return hipGetLastError();
 }
 
