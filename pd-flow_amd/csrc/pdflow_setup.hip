@@ -435,8 +435,9 @@ __device__ __forceinline__ void pyramid_body(const PyrArgs &p, unsigned tile)
     const bool inner = (v > 0) && (v < (int)rows_i - 1) && (u > 0) && (u < (int)cols_i - 1);
     if (inner) {
         // every tap is in range.  A rejected depth tap adds +0 to wd and sumd instead of
-        // being skipped: both sums start at +0 and only add positive terms, so they are
-        // never -0 and adding +0 leaves them unchanged.
+        // being skipped.  Both sums start at +0, and under round-to-nearest a sum only reaches
+        // zero by exact cancellation, which gives +0: they are never -0, so adding +0 leaves
+        // them unchanged whatever the mask and depths.
 #pragma unroll
         for (int k = -2; k < 3; k++)
 #pragma unroll
