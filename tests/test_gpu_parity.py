@@ -313,3 +313,30 @@ def test_copy_all_solutions_per_level(gpu_available):
         for n, kind in (("depth", "depth"), ("depth_old", "depth_old"), ("colour", "colour"),
                         ("xx_old", "xx_old"), ("yy", "yy")):
             np.testing.assert_array_equal(got[n], o.get_pyramid(kind, limg))
+
+
+def test_non_default_parameters_match_the_oracle(gpu_available):
+    """Every parameter changed from the pair evaluator's (lambda_i, lambda_d, mu, a lopsided 5x5
+    mask whose rows and columns differ, fields of view): the single-pair solve equals the
+    oracle's, and a batch of two pairs equals the single-pair solves bit for bit."""
+    from pdflow import Batch
+    mask = np.outer([1.0, 2.0, 6.0, 3.0, 1.0], [2.0, 3.0, 5.0, 4.0, 1.0]).astype(np.float32)
+    mask /= mask.sum()
+    over = dict(lambda_i=0.07, lambda_d=0.2, mu=40.0, g_mask=mask, fovh=1.1, fovv=0.85)
+    rows, cam = 240, 2
+    say("non-default parameters: 240x320 pair against the oracle, then a batch of two")
+    pairs = [synth_pair_float(s, 240, 320) for s in (77, 78)]
+    o = Oracle(rows, cam, threads=8, **over)
+    want = o.solve_pair(*pairs[0])
+    g = Solver(rows, cam, **over)
+    single = [g.solve_pair(*p) for p in pairs]
+    for a, b, n in zip(single[0], want, ("dx", "dy", "dz")):
+        assert_same(a, b, n)
+    b = Batch([rows, rows], **over)
+    for i, p in enumerate(pairs):
+        b.set_inputs(i, *p)
+    b.solve(graph=True)
+    b.synchronize()
+    for i in range(2):
+        for a, w in zip(b.flow(i), single[i]):
+            np.testing.assert_array_equal(a, w, err_msg=f"batch pair {i}")
