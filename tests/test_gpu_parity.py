@@ -340,3 +340,46 @@ def test_non_default_parameters_match_the_oracle(gpu_available):
     for i in range(2):
         for a, w in zip(b.flow(i), single[i]):
             np.testing.assert_array_equal(a, w, err_msg=f"batch pair {i}")
+
+
+def _hard_pairs():
+    """(name, I1, Z1, I2, Z2) at 120x160: a large depth hole plus striped holes, constant and
+    equal intensities (zero image gradients everywhere), sharp depth steps from 0.3 to 10 m"""
+    I1, Z1, I2, Z2 = synth_pair_float(91, 120, 160)
+    out = []
+    h1, h2 = Z1.copy().reshape(160, 120), Z2.copy().reshape(160, 120)   # column-major planes
+    h1[40:90, 20:70] = 0.0
+    h2[50:100, 30:80] = 0.0
+    h1[:, ::7] = 0.0
+    h2[::11, :] = 0.0
+    out.append(("holes", I1, h1.ravel().copy(), I2, h2.ravel().copy()))
+    flat = np.full_like(I1, 128.0)
+    out.append(("flat intensity", flat, Z1, flat.copy(), Z2))
+    s1, s2 = Z1.copy().reshape(160, 120), Z2.copy().reshape(160, 120)
+    s1[::2, :] = np.where(s1[::2, :] > 0, 0.3, 0.0)
+    s1[1::2, :] = np.where(s1[1::2, :] > 0, 10.0, 0.0)
+    s2[:, ::3] = np.where(s2[:, ::3] > 0, 9.5, 0.0)
+    out.append(("depth steps", I1, s1.ravel().copy(), I2, s2.ravel().copy()))
+    return out
+
+
+def test_hard_inputs_match_the_oracle(gpu_available):
+    """Holes, textureless images and depth steps take the border and rejection branches of the
+    pyramid, gradients, warp, prox and median filter: the flow equals the oracle's bit for bit,
+    single and batched."""
+    from pdflow import Batch
+    pairs = _hard_pairs()
+    g, o = Solver(120, 4), Oracle(120, 4, threads=8)
+    b = Batch([120] * len(pairs))
+    for i, (name, *p) in enumerate(pairs):
+        say(f"hard input: {name}")
+        got = g.solve_pair(*p)
+        want = o.solve_pair(*p)
+        for a, w, n in zip(got, want, ("dx", "dy", "dz")):
+            np.testing.assert_array_equal(a, w, err_msg=f"{name} {n}")
+        b.set_inputs(i, *p)
+    b.solve(graph=True)
+    b.synchronize()
+    for i, (name, *p) in enumerate(pairs):
+        for a, w, n in zip(b.flow(i), o.solve_pair(*p), ("dx", "dy", "dz")):
+            np.testing.assert_array_equal(a, w, err_msg=f"batched {name} {n}")
