@@ -170,3 +170,22 @@ def test_autotune_picks_a_config_and_caches_it(gpu_available, tmp_path):
         b2.set_inputs(i, *single(r, 900 + i)[0])
     assert b2.autotune(cache)[:2] == (shape, k)
     assert len(cache.read_text().splitlines()) == 1
+
+
+def test_full_size_batch_matches_single_pairs(gpu_available):
+    """256 VGA pairs in one batch, the benchmark's size (a 141557760-thread iteration grid, about
+    12 GB of planes): sampled pairs, the last one included, equal their single-pair solves."""
+    from pdflow import synth_pairs_float
+    n, rows = 256, 480
+    say(f"{n} pairs of {rows}x640 in one batch")
+    planes = synth_pairs_float(4200, n, rows, 640)
+    b = Batch([rows] * n)
+    for i in range(n):
+        b.set_inputs(i, *(p[i] for p in planes))
+    b.solve(graph=True)
+    b.synchronize()
+    g = Solver(rows, 1)
+    for i in (0, 97, 200, n - 1):
+        want = g.solve_pair(*(p[i] for p in planes))
+        for a, w in zip(b.flow(i), want):
+            np.testing.assert_array_equal(a, w, err_msg=f"pair {i}")
