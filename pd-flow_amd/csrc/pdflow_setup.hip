@@ -42,13 +42,14 @@ __host__ __device__ inline unsigned tiles_of(unsigned rows, unsigned cols)
 }
 
 // the coarse patch a tile's 5x5 even taps touch: coarse rows v0/2-1 .. v0/2+16, columns
-// u0/2-1 .. u0/2+4 (clamped to the coarse level), at most 18 x 6, stored with row stride kCR
-constexpr int kCR = kTR / 2 + 3, kCC = kTC / 2 + 3;
+// u0/2-1 .. u0/2+4 (clamped to the coarse level), at most 18 x 6, stored with row stride kCR;
+// element kZero of each plane holds +0 for the taps that fall outside the level
+constexpr int kCR = kTR / 2 + 3, kCC = kTC / 2 + 3, kZero = kCR * kCC;
 static_assert(kCR * kCC <= kBlock, "one coarse element per thread and plane");
 
 __device__ __forceinline__ void level_init_body(const LevelArgs &a, unsigned tile, int has_coarse)
 {
-    __shared__ float sC[10][kCR * kCC];
+    __shared__ float sC[10][kZero + 1];
     const unsigned rows_i = a.rows, cols_i = a.cols;
     const unsigned tv = (rows_i + kTR - 1) / kTR;
     const int v0 = (int)(tile % tv) * kTR, u0 = (int)(tile / tv) * kTC;
@@ -68,21 +69,44 @@ __device__ __forceinline__ void level_init_body(const LevelArgs &a, unsigned til
 #pragma unroll
             for (int q = 0; q < 10; q++) sC[q][t] = gptr(a.f[kCarry[q]])[src];
         }
+        if (t < 10) sC[t][kZero] = 0.f;
         __syncthreads();
         if (on) {
             // .cu:460-569: the zero-inserted plane holds the coarse value at even (v, u) (du, dv
             // doubled) and +0 elsewhere.  A running sum that starts at +0 is never -0 under
-            // round-to-nearest, so adding the +0 products of odd taps leaves it unchanged: only
-            // the even taps are visited, in the reference's (k outer, l inner) order.
+            // round-to-nearest, and adding +0 or -0 to it leaves it unchanged: so only the even
+            // taps are visited, in the reference's (k outer, l inner) order, and a tap outside
+            // the level (or the third tap of an odd row or column) reads the +0 element, whose
+            // product with the finite weight is such a zero.  Rows and columns of one parity
+            // visit k (l) = -2, 0, 2, the other -1, 1: the loops are unrolled over the three
+            // tap slots and each lane picks its weight from the four parity classes, read once
+            // per workgroup, so every lane runs the same instructions.
             const bool inner = (v > 1) && (v < (int)rows_i - 2) && (u > 1) && (u < (int)cols_i - 2);
-            for (int k = (v & 1) ? -1 : -2; k < 3; k += 2) {
-                const int vv = v + k;
-                if (vv < 0 || vv >= (int)rows_i) continue;
-                for (int l = (u & 1) ? -1 : -2; l < 3; l += 2) {
-                    const int uu = u + l;
-                    if (uu < 0 || uu >= (int)cols_i) continue;
-                    const float g = 4.f * a.g[12 + k + 5 * l];
-                    const int ci = (vv / 2 - cr0) + (uu / 2 - cc0) * kCR;
+            const int vo = v & 1, uo = u & 1;
+            // the 25 weights as wave-uniform values, so that a per-lane choice between two of
+            // them selects registers rather than addresses
+            float gw[25];
+#pragma unroll
+            for (int n = 0; n < 25; n++)
+                gw[n] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(a.g[n])));
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int k = 2 * j - 2 + vo, vv = v + k;
+                const bool vin = (j < 2 || vo == 0) && vv >= 0 && vv < (int)rows_i;
+                const int ro = min(max((vv >> 1) - cr0, 0), kCR - 1);
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const int l = 2 * i - 2 + uo, uu = u + l;
+                    const bool tin = vin && (i < 2 || uo == 0) && uu >= 0 && uu < (int)cols_i;
+                    const int co = min(max((uu >> 1) - cc0, 0), kCC - 1);
+                    const int ci = tin ? ro + co * kCR : kZero;
+                    // weights of tap slot (j, i) for even/odd v and u (k, l clamped to 2 where
+                    // the slot does not exist; its product is zero there)
+                    const int ke = 2 * j - 2, ko = min(2 * j - 1, 2);
+                    const int le = 2 * i - 2, lo = min(2 * i - 1, 2);
+                    const float gv_e = uo ? gw[12 + ke + 5 * lo] : gw[12 + ke + 5 * le];
+                    const float gv_o = uo ? gw[12 + ko + 5 * lo] : gw[12 + ko + 5 * le];
+                    const float g = 4.f * (vo ? gv_o : gv_e);
 #pragma unroll
                     for (int q = 0; q < 10; q++) {
                         const float cval = sC[q][ci];
