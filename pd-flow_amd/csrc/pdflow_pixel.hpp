@@ -192,14 +192,17 @@ __device__ __forceinline__ Corners corners(const float *plane, unsigned rows_i, 
 
 __device__ __forceinline__ float bilinear(const Corners &c, const Tap &t)
 {
-    // interpolatePixel (.cu:666-681) after clamping
+    // interpolatePixel (.cu:666-681) after clamping.  Every case is evaluated with the
+    // reference's operations and one is selected, so that no branch holds a corner load:
+    // the corners of all the planes a kernel gathers are then in flight together.
     const bool eu = t.su == t.fu, ev = t.sv == t.fv;
-    if (eu && ev) return c.ff;
-    if (eu) return ((float)t.sv - t.iv) * c.ff + (t.iv - (float)t.fv) * c.sf;
-    if (ev) return ((float)t.su - t.iu) * c.ff + (t.iu - (float)t.fu) * c.fs;
-    const float top = ((float)t.su - t.iu) * c.sf + (t.iu - (float)t.fu) * c.ss;
-    const float bot = ((float)t.su - t.iu) * c.ff + (t.iu - (float)t.fu) * c.fs;
-    return ((float)t.sv - t.iv) * bot + (t.iv - (float)t.fv) * top;
+    const float au = (float)t.su - t.iu, bu = t.iu - (float)t.fu;
+    const float av = (float)t.sv - t.iv, bv = t.iv - (float)t.fv;
+    const float col = av * c.ff + bv * c.sf;       // su == fu
+    const float bot = au * c.ff + bu * c.fs;       // sv == fv, and the general case's bottom row
+    const float top = au * c.sf + bu * c.ss;
+    const float both = av * bot + bv * top;
+    return eu ? (ev ? c.ff : col) : (ev ? bot : both);
 }
 
 __device__ __forceinline__ float bilinear(const float *plane, unsigned rows_i, const Tap &t)
@@ -207,15 +210,28 @@ __device__ __forceinline__ float bilinear(const float *plane, unsigned rows_i, c
     return bilinear(corners(plane, rows_i, t), t);
 }
 
-__device__ __forceinline__ float bilinear_depth(const float *plane, unsigned rows_i, const Tap &t)
+// An empty statement that takes the four corners as operands and clobbers memory: their loads
+// stay where they were issued (not sunk into the branches that a later select may become),
+// and no load issued before it moves below it.  Placed after all of a kernel's gathers, it
+// lets them all be in flight at once.
+__device__ __forceinline__ void issued(const Corners &c)
+{
+    asm volatile("" ::"v"(c.ff), "v"(c.sf), "v"(c.fs), "v"(c.ss) : "memory");
+}
+
+__device__ __forceinline__ float bilinear_depth(const Corners &c, const Tap &t)
 {
     // interpolatePixelDepth (.cu:696-720): nearest neighbour (half to even) if a tap is 0
-    const Corners c = corners(plane, rows_i, t);
-    if (c.ss == 0.f || c.sf == 0.f || c.fs == 0.f || c.ff == 0.f) {
-        const unsigned ru = (unsigned)rintf(t.iu), rv = (unsigned)rintf(t.iv);
-        return ru == t.fu ? (rv == t.fv ? c.ff : c.sf) : (rv == t.fv ? c.fs : c.ss);
-    }
-    return bilinear(c, t);
+    const bool hole = c.ss == 0.f || c.sf == 0.f || c.fs == 0.f || c.ff == 0.f;
+    const unsigned ru = (unsigned)rintf(t.iu), rv = (unsigned)rintf(t.iv);
+    const float nearest = ru == t.fu ? (rv == t.fv ? c.ff : c.sf) : (rv == t.fv ? c.fs : c.ss);
+    const float interp = bilinear(c, t);
+    return hole ? nearest : interp;
+}
+
+__device__ __forceinline__ float bilinear_depth(const float *plane, unsigned rows_i, const Tap &t)
+{
+    return bilinear_depth(corners(plane, rows_i, t), t);
 }
 
 // Stable sort of 9 (key, weight) pairs by key.  The reference uses a bubble sort with a

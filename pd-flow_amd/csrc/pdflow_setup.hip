@@ -239,22 +239,33 @@ __device__ __forceinline__ void warp_mu_body(const LevelArgs &a, unsigned block,
     const float iu = (float)u + gptr(a.f[PDF_DU_PREV])[index];
     const float iv = (float)v + gptr(a.f[PDF_DV_PREV])[index];
     const Tap t = make_tap(rows_i, cols_i, iu, iv);
-    const float dct = bilinear(a.I, rows_i, t) - gptr(a.Iold)[index];
-    const float dcu = bilinear(a.f[PDF_DCU_AUX], rows_i, t);
-    const float dcv = bilinear(a.f[PDF_DCV_AUX], rows_i, t);
-    const float wd = bilinear_depth(a.Z, rows_i, t);
-    const float ddt = (wd > 0.f) ? wd - gptr(a.Zold)[index] : 0.f;
-    const float ddu = bilinear(a.f[PDF_DDU_AUX], rows_i, t);
-    const float ddv = bilinear(a.f[PDF_DDV_AUX], rows_i, t);
-    gptr(a.f[PDF_DCT])[index] = dct;
-    gptr(a.f[PDF_DCU])[index] = dcu;
-    gptr(a.f[PDF_DCV])[index] = dcv;
-    gptr(a.f[PDF_DDT])[index] = ddt;
-    gptr(a.f[PDF_DDU])[index] = ddu;
-    gptr(a.f[PDF_DDV])[index] = ddv;
+    // the 24 corners of the six gathered planes and the two frame-1 values are all issued
+    // before any of them is used; the output planes and mu are read before that point too
+    const Corners cI = corners(a.I, rows_i, t), cCU = corners(a.f[PDF_DCU_AUX], rows_i, t),
+                  cCV = corners(a.f[PDF_DCV_AUX], rows_i, t), cZ = corners(a.Z, rows_i, t),
+                  cDU = corners(a.f[PDF_DDU_AUX], rows_i, t), cDV = corners(a.f[PDF_DDV_AUX], rows_i, t);
+    const float iold = gptr(a.Iold)[index], zold = gptr(a.Zold)[index];
+    float *const o_ct = a.f[PDF_DCT], *const o_cu = a.f[PDF_DCU], *const o_cv = a.f[PDF_DCV];
+    float *const o_dt = a.f[PDF_DDT], *const o_du = a.f[PDF_DDU], *const o_dv = a.f[PDF_DDV];
+    float *const o_mu = a.f[PDF_MU_UV];
+    const float mu = a.mu;
+    issued(cI); issued(cCU); issued(cCV); issued(cZ); issued(cDU); issued(cDV);
+    const float dct = bilinear(cI, t) - iold;
+    const float dcu = bilinear(cCU, t);
+    const float dcv = bilinear(cCV, t);
+    const float wd = bilinear_depth(cZ, t);
+    const float ddt = (wd > 0.f) ? wd - zold : 0.f;
+    const float ddu = bilinear(cDU, t);
+    const float ddv = bilinear(cDV, t);
+    gptr(o_ct)[index] = dct;
+    gptr(o_cu)[index] = dcu;
+    gptr(o_cv)[index] = dcv;
+    gptr(o_dt)[index] = ddt;
+    gptr(o_du)[index] = ddu;
+    gptr(o_dv)[index] = ddv;
     // K8 computeMu (.cu:781-784)
-    const float mu_uv = a.mu / (1.f + 1000.f * (ddu * ddu + ddv * ddv + ddt * ddt));
-    gptr(a.f[PDF_MU_UV])[index] = mu_uv;
+    const float mu_uv = mu / (1.f + 1000.f * (ddu * ddu + ddv * ddv + ddt * ddt));
+    gptr(o_mu)[index] = mu_uv;
     if (write_steps) {
         // K8 computeStepSizes (.cu:786-808), needed only by the unfused iteration bridges
         const float ri = gptr(a.f[PDF_RI])[index], rj = gptr(a.f[PDF_RJ])[index];
