@@ -79,6 +79,7 @@ struct Launch {
     int flag;
     size_t args_off, first_off;
     unsigned n, blocks;
+    unsigned stride;        // blocks per item when every item has the same number, else 0
     uint64_t pixel_iters;   // PD launches: pixel-iterations the launch performs
 };
 
@@ -235,6 +236,9 @@ void add_launch(Plan &plan, Launch l, const std::vector<T> &args, const std::vec
     l.first_off = put(first.data(), first.size() * sizeof(unsigned));
     l.n = (unsigned)args.size();
     l.blocks = first.back();
+    l.stride = blocks[0];
+    for (unsigned x : blocks)
+        if (x != blocks[0]) l.stride = 0;
     plan.launches.push_back(l);
 }
 
@@ -477,7 +481,7 @@ int ensure_plan(pdf_batch *b)
 int run_plan(pdf_batch *b, const Plan &plan, char *dev_tables, bool timed)
 {
     for (const Launch &l : plan.launches) {
-        pdf::BatchTable t{dev_tables + l.args_off, (const unsigned *)(dev_tables + l.first_off), l.n};
+        pdf::BatchTable t{dev_tables + l.args_off, (const unsigned *)(dev_tables + l.first_off), l.n, l.stride};
         if (l.kind == Launch::Net) {
             const int rc = pdf::cnn::net_forward_on(b->net, b->learn_x, l.n, b->stream);
             if (rc) return bfail(b, rc, std::string("network forward: ") + pdf_net_last_error(b->net));
@@ -1065,7 +1069,7 @@ int metrics_of(pdf_batch *b, const std::vector<unsigned> &who, float *energy, fl
     char *tables = nullptr, *extra = nullptr;
     int rc = upload_scratch_plan(b, m, 3 * sizeof(float) * L.blocks, &tables, &extra);
     if (rc) return rc;
-    pdf::BatchTable t{tables + L.args_off, (const unsigned *)(tables + L.first_off), L.n};
+    pdf::BatchTable t{tables + L.args_off, (const unsigned *)(tables + L.first_off), L.n, L.stride};
     hipError_t e = pdf::launch_learn(L.learn, t, L.blocks, (float *)extra, b->stream);
     if (e != hipSuccess) return bfail(b, PDF_ERR_HIP, std::string("metrics launch: ") + hipGetErrorString(e));
     std::vector<float> part(3 * (size_t)L.blocks);
@@ -1131,7 +1135,7 @@ int pdf_batch_epe(pdf_batch *b, const pdf_batch *ref, float *epe)
     BHIP(b, hipStreamSynchronize(ref->stream));
     int rc = upload_scratch_plan(b, m, 2 * sizeof(float) * L.blocks, &tables, &extra);
     if (rc) return rc;
-    pdf::BatchTable t{tables + L.args_off, (const unsigned *)(tables + L.first_off), L.n};
+    pdf::BatchTable t{tables + L.args_off, (const unsigned *)(tables + L.first_off), L.n, L.stride};
     hipError_t e = pdf::launch_learn(L.learn, t, L.blocks, (float *)extra, b->stream);
     if (e != hipSuccess) return bfail(b, PDF_ERR_HIP, std::string("EPE launch: ") + hipGetErrorString(e));
     std::vector<float> part(2 * (size_t)L.blocks);
@@ -1206,7 +1210,7 @@ int pdf_batch_learn_samples(pdf_batch *b, unsigned h, unsigned w, float *x_dev, 
     int rc = upload_scratch_plan(b, m, 0, &tables, nullptr);
     if (rc) return rc;
     const Launch &L = m.launches[0];
-    pdf::BatchTable t{tables + L.args_off, (const unsigned *)(tables + L.first_off), L.n};
+    pdf::BatchTable t{tables + L.args_off, (const unsigned *)(tables + L.first_off), L.n, L.stride};
     hipError_t e = pdf::launch_learn(pdf::LearnKernel::GatherInputs, t, L.blocks, nullptr, b->stream);
     if (e == hipSuccess) e = pdf::launch_learn(pdf::LearnKernel::Labels, t, L.blocks, nullptr, b->stream);
     if (e != hipSuccess) return bfail(b, PDF_ERR_HIP, std::string("sample launch: ") + hipGetErrorString(e));

@@ -61,6 +61,7 @@ struct BatchTable {
     const void *args;
     const unsigned *first;
     unsigned n;
+    unsigned stride;   // blocks of every item when all items have the same number, else 0
 };
 
 struct BatchItem {
@@ -89,9 +90,11 @@ __device__ __forceinline__ unsigned xcd_swizzle(unsigned b, unsigned n)
     return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
-// the item that owns block b: largest i with first[i] <= b (binary search)
+// the item that owns block b: b / stride when every item has `stride` blocks, else the largest
+// i with first[i] <= b (binary search: one dependent table load per halving)
 __device__ __forceinline__ BatchItem batch_item(const BatchTable &t, unsigned b)
 {
+    if (t.stride) return {b / t.stride, b % t.stride};
     unsigned lo = 0, hi = t.n;            // invariant: first[lo] <= b < first[hi]
     while (hi - lo > 1) {
         const unsigned mid = (lo + hi) / 2;

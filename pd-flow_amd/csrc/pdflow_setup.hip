@@ -451,14 +451,25 @@ __device__ __forceinline__ void pyramid_body(const PyrArgs &p, unsigned tile)
     const unsigned index = (unsigned)v + (unsigned)u * rows_i;   // read only when on
     const int pr = 2 * (int)rows_i, pc = 2 * (int)cols_i;   // the finer level
     const int r0 = 2 * v0 - 2, c0 = 2 * u0 - 2;
+    // every thread's patch loads are issued before the first is stored: coordinates outside
+    // the finer level are clamped into it, and the cells they fill are never read (an inner
+    // pixel's taps are all in range, a border pixel skips the taps outside)
+    constexpr int kLoads = (kPatch + kBlock - 1) / kBlock;
+    float lc[kLoads], ld[kLoads];
 #pragma unroll
-    for (int i = 0; i < (kPatch + kBlock - 1) / kBlock; i++) {
+    for (int i = 0; i < kLoads; i++) {
         const int e = (int)threadIdx.x + i * kBlock;
-        const int vv = r0 + e % kPR, uu = c0 + e / kPR;
-        if (e < kPatch && vv >= 0 && vv < pr && uu >= 0 && uu < pc) {
-            const unsigned ip = (unsigned)vv + (unsigned)uu * (unsigned)pr;
-            sCol[e] = gptr(p.colp)[ip];
-            sDep[e] = gptr(p.depp)[ip];
+        const int vv = min(max(r0 + e % kPR, 0), pr - 1), uu = min(max(c0 + e / kPR, 0), pc - 1);
+        const unsigned ip = (unsigned)vv + (unsigned)uu * (unsigned)pr;
+        lc[i] = gptr(p.colp)[ip];
+        ld[i] = gptr(p.depp)[ip];
+    }
+#pragma unroll
+    for (int i = 0; i < kLoads; i++) {
+        const int e = (int)threadIdx.x + i * kBlock;
+        if (e < kPatch) {
+            sCol[e] = lc[i];
+            sDep[e] = ld[i];
         }
     }
     __syncthreads();
