@@ -346,7 +346,7 @@ void build_plan(pdf_batch *b, const std::vector<unsigned> &who, bool fallback, P
             if (l >= p.pyr_levels) continue;
             for (int fr = 0; fr < 2; fr++) {
                 args.push_back(pyr_args(b, p, l, fr == 0));
-                blocks.push_back(pdf::batch_blocks(pdf::BatchKernel::Pyramid, args.back().rows_i, args.back().cols_i));
+                blocks.push_back(pdf::batch_pyramid_blocks(l, args.back().rows_i, args.back().cols_i));
             }
         }
         add_launch(plan, setup_launch(pdf::BatchKernel::Pyramid, 0), args, blocks);

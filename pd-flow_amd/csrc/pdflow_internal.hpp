@@ -182,6 +182,8 @@ unsigned batch_blocks_per_level(unsigned pixels);
 // blocks a batched set-up kernel takes for one item of rows x cols (the 2-D tiled kernels
 // count tiles, the others 256-pixel blocks)
 unsigned batch_blocks(BatchKernel k, unsigned rows, unsigned cols);
+// blocks of one frame's pyramid level in a batched pyramid launch
+unsigned batch_pyramid_blocks(unsigned level, unsigned rows, unsigned cols);
 // k = 0: whole-level kernel (iters from PDArgs), 1: one iteration, >1: temporal blocking
 hipError_t launch_pd_batch(int k, int shape, const BatchTable &t, unsigned total_blocks, hipStream_t s);
 unsigned pd_blocks(int k, int shape, unsigned rows, unsigned cols);

@@ -22,6 +22,7 @@ namespace pdf {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kPyr0PerThread = 4;   // level-0 pixels per thread of the batched pyramid
 
 
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
@@ -423,6 +424,10 @@ __device__ __forceinline__ void filter_motion_body(const LevelArgs &a, unsigned 
 constexpr int kPR = 2 * kTR + 3, kPC = 2 * kTC + 3;   // 67 x 19 patch
 constexpr int kPatch = kPR * kPC;
 
+// Level 0 is a copy with PX0 pixels per thread: block `tile` covers pixels
+// [tile * PX0 * kBlock, (tile + 1) * PX0 * kBlock) in column-major order, one contiguous run of
+// kBlock pixels per pass, so that a wave's accesses are 256 contiguous bytes.
+template <int PX0>
 __device__ __forceinline__ void pyramid_body(const PyrArgs &p, unsigned tile)
 {
     __shared__ float sCol[kPatch], sDep[kPatch];
@@ -431,17 +436,28 @@ __device__ __forceinline__ void pyramid_body(const PyrArgs &p, unsigned tile)
     const float disp_u = 0.5f * (float)(cols_i - 1);
     const float disp_v = 0.5f * (float)(rows_i - 1);
     if (p.level == 0) {
-        // a copy: pixels in column-major order (the tiles outnumber rows_i * cols_i / kBlock),
-        // so that a wave's accesses are 256 contiguous bytes.  A batch's level-0 colour and
-        // depth planes are its input planes themselves, and only xx, yy are written.
-        const unsigned index = tile * kBlock + threadIdx.x;
-        if (index >= rows_i * cols_i) return;
-        const unsigned v = index % rows_i, u = index / rows_i;
-        const float depth = gptr(p.depth_wf)[index];
-        if (p.col != p.colour_wf) gptr(p.col)[index] = gptr(p.colour_wf)[index];
-        if (p.dep != p.depth_wf) gptr(p.dep)[index] = depth;
-        gptr(p.xx)[index] = ((float)u - disp_u) * depth * p.inv_f_i;
-        gptr(p.yy)[index] = ((float)v - disp_v) * depth * p.inv_f_i;
+        // A batch's level-0 colour and depth planes are its input planes themselves, and only
+        // xx, yy are written.  The loads of all passes are issued first (at an index clamped
+        // into the plane; a pass past the end stores nothing).
+        const unsigned P = rows_i * cols_i;
+        float depth[PX0], colour[PX0];
+        const bool copy_col = p.col != p.colour_wf, copy_dep = p.dep != p.depth_wf;
+#pragma unroll
+        for (int i = 0; i < PX0; i++) {
+            const unsigned index = min((tile * PX0 + i) * kBlock + threadIdx.x, P - 1);
+            depth[i] = gptr(p.depth_wf)[index];
+            colour[i] = copy_col ? gptr(p.colour_wf)[index] : 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < PX0; i++) {
+            const unsigned index = (tile * PX0 + i) * kBlock + threadIdx.x;
+            if (index >= P) break;
+            const unsigned v = index % rows_i, u = index / rows_i;
+            if (copy_col) gptr(p.col)[index] = colour[i];
+            if (copy_dep) gptr(p.dep)[index] = depth[i];
+            gptr(p.xx)[index] = ((float)u - disp_u) * depth[i] * p.inv_f_i;
+            gptr(p.yy)[index] = ((float)v - disp_v) * depth[i] * p.inv_f_i;
+        }
         return;
     }
     const unsigned tv = (rows_i + kTR - 1) / kTR;
@@ -530,7 +546,7 @@ __global__ void __launch_bounds__(kBlock) k_rij_gradients(LevelArgs a) { rij_gra
 __global__ void __launch_bounds__(kBlock) k_warm_init(LevelArgs a) { warm_init_body(a, blockIdx.x); }
 __global__ void __launch_bounds__(kBlock) k_warp_mu(LevelArgs a, int write_steps) { warp_mu_body(a, blockIdx.x, write_steps); }
 __global__ void __launch_bounds__(kBlock) k_filter_motion(LevelArgs a) { filter_motion_body(a, blockIdx.x); }
-__global__ void __launch_bounds__(kBlock) k_pyramid2(Pyr2Args a) { pyramid_body(a.frame[blockIdx.y], blockIdx.x); }
+__global__ void __launch_bounds__(kBlock) k_pyramid2(Pyr2Args a) { pyramid_body<1>(a.frame[blockIdx.y], blockIdx.x); }
 
 __global__ void __launch_bounds__(kBlock) kb_level_init(BatchTable t, int has_coarse)
 {
@@ -561,7 +577,7 @@ __global__ void __launch_bounds__(kBlock) kb_filter_motion(BatchTable t, int fla
 __global__ void __launch_bounds__(kBlock) kb_pyramid(BatchTable t)
 {
     const BatchItem it = batch_item(t);
-    pyramid_body(static_cast<const PyrArgs *>(t.args)[it.item], it.block);
+    pyramid_body<kPyr0PerThread>(static_cast<const PyrArgs *>(t.args)[it.item], it.block);
 }
 
 }  // namespace
@@ -634,6 +650,12 @@ hipError_t launch_batch(BatchKernel k, const BatchTable &t, unsigned total_block
 }
 
 unsigned batch_blocks_per_level(unsigned pixels) { return (pixels + kBlock - 1) / kBlock; }
+
+unsigned batch_pyramid_blocks(unsigned level, unsigned rows, unsigned cols)
+{
+    if (level == 0) return (rows * cols + kPyr0PerThread * kBlock - 1) / (kPyr0PerThread * kBlock);
+    return tiles_of(rows, cols);
+}
 
 unsigned batch_blocks(BatchKernel k, unsigned rows, unsigned cols)
 {
