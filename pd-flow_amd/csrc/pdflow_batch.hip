@@ -274,7 +274,7 @@ pdf::LearnArgs learn_args(const pdf_batch *b, const Pair &p)
 // are consumed in rounds; one launch per round and launch size covers every pair whose next
 // launch has that size.
 template <class N>
-void add_iterations(pdf_batch *b, Plan &plan, const std::vector<unsigned> &active, N n_of)
+std::vector<std::vector<int>> iteration_chunks(const pdf_batch *b, const std::vector<unsigned> &active, N n_of)
 {
     const unsigned P = active.empty() ? 0 : b->pairs[active[0]].rows_i * b->pairs[active[0]].cols_i;
     const bool whole = b->level_kernel && pdf::pd_level_supported(P);
@@ -285,17 +285,33 @@ void add_iterations(pdf_batch *b, Plan &plan, const std::vector<unsigned> &activ
         if (whole) { if (n) todo[j].push_back(0); }
         else todo[j] = pdf::tb_chunks(n, K);
     }
+    return todo;
+}
+
+// fresh[j] (if given): the level-init of active[j] left the constant state planes to its first
+// launch, which is temporally blocked; launches are grouped by size and by that property
+template <class N>
+void add_iterations(pdf_batch *b, Plan &plan, const std::vector<unsigned> &active, N n_of,
+                    const std::vector<char> &fresh = {})
+{
+    const std::vector<std::vector<int>> todo = iteration_chunks(b, active, n_of);
     std::vector<size_t> pos(active.size(), 0);
+    auto fresh_next = [&](size_t j) { return !fresh.empty() && fresh[j] && pos[j] == 0; };
     for (;;) {
         int k = -1;
-        for (size_t j = 0; j < active.size() && k < 0; j++)
-            if (pos[j] < todo[j].size()) k = todo[j][pos[j]];
+        bool fr = false;
+        for (size_t j = 0; j < active.size(); j++)
+            if (pos[j] < todo[j].size()) {
+                k = todo[j][pos[j]];
+                fr = fresh_next(j);
+                break;
+            }
         if (k < 0) break;
         std::vector<pdf::PDArgs> args;
         std::vector<unsigned> blocks;
         uint64_t pix = 0;
         for (size_t j = 0; j < active.size(); j++) {
-            if (pos[j] >= todo[j].size() || todo[j][pos[j]] != k) continue;
+            if (pos[j] >= todo[j].size() || todo[j][pos[j]] != k || fresh_next(j) != fr) continue;
             Pair &p = b->pairs[active[j]];
             const unsigned n = k == 0 ? n_of(p) : (unsigned)k;
             args.push_back(pd_args(b, p, n));
@@ -307,6 +323,7 @@ void add_iterations(pdf_batch *b, Plan &plan, const std::vector<unsigned> &activ
         Launch l{};
         l.kind = Launch::PD;
         l.k = k;
+        l.flag = fr ? 1 : 0;
         l.pixel_iters = pix;
         add_launch(plan, l, args, blocks);
     }
@@ -375,20 +392,35 @@ void build_plan(pdf_batch *b, const std::vector<unsigned> &who, bool fallback, P
         }
     }
     for (unsigned s = 0; s < max_steps; s++) {
-        std::vector<unsigned> active, warm_now, cold_now;
+        std::vector<unsigned> active, warm_now;
+        std::vector<char> warm;
         for (unsigned i : who) {
             Pair &p = b->pairs[i];
             const unsigned st = start_of(p);
             if (s >= p.ctf_levels || s < st) continue;
             level_dims(p, s, &p.rows_i, &p.cols_i, &p.level_image);
             active.push_back(i);
-            (st > 0 && s == st ? warm_now : cold_now).push_back(i);
+            warm.push_back(st > 0 && s == st);
+            if (warm.back()) warm_now.push_back(i);
         }
-        add_setup(b, plan, pdf::BatchKernel::LevelInit, s > 0 ? 1 : 0, cold_now);
+        auto n_of = [&](const Pair &p) { return iters_of(p, s); };
+        // a cold pair whose first launch at this level is temporally blocked gets a fresh first
+        // launch, and its level-init does not write the constant state planes
+        const std::vector<std::vector<int>> chunks = iteration_chunks(b, active, n_of);
+        std::vector<char> fresh(active.size(), 0);
+        std::vector<unsigned> cold_fresh, cold_full;
+        for (size_t j = 0; j < active.size(); j++) {
+            if (warm[j]) continue;
+            fresh[j] = !chunks[j].empty() && chunks[j][0] > 1;
+            (fresh[j] ? cold_fresh : cold_full).push_back(active[j]);
+        }
+        const int coarse = s > 0 ? pdf::kInitCoarse : 0;
+        add_setup(b, plan, pdf::BatchKernel::LevelInit, coarse | pdf::kInitSkipConst, cold_fresh);
+        add_setup(b, plan, pdf::BatchKernel::LevelInit, coarse, cold_full);
         add_setup(b, plan, pdf::BatchKernel::WarmInit, 0, warm_now);
         add_setup(b, plan, pdf::BatchKernel::RijGradients, 0, active);
         add_setup(b, plan, pdf::BatchKernel::WarpMu, 0, active);
-        add_iterations(b, plan, active, [&](const Pair &p) { return iters_of(p, s); });
+        add_iterations(b, plan, active, n_of, fresh);
         std::vector<unsigned> finest, inner;
         for (unsigned i : active) (s + 1 == b->pairs[i].ctf_levels ? finest : inner).push_back(i);
         add_setup(b, plan, pdf::BatchKernel::FilterMotion, 0, inner);
@@ -497,7 +529,7 @@ int run_plan(pdf_batch *b, const Plan &plan, char *dev_tables, bool timed)
         hipError_t e;
         switch (l.kind) {
         case Launch::Setup: e = pdf::launch_batch(l.setup, t, l.blocks, l.flag, b->stream); break;
-        case Launch::PD: e = pdf::launch_pd_batch(l.k, b->tb_shape, t, l.blocks, b->stream); break;
+        case Launch::PD: e = pdf::launch_pd_batch(l.k, b->tb_shape, l.flag != 0, t, l.blocks, b->stream); break;
         default: e = pdf::launch_learn(l.learn, t, l.blocks, nullptr, b->stream); break;
         }
         if (e != hipSuccess) return bfail(b, PDF_ERR_HIP, std::string("batched launch: ") + hipGetErrorString(e));

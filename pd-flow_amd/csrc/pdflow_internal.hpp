@@ -110,6 +110,10 @@ enum class BatchKernel { LevelInit, RijGradients, WarpMu, FilterMotion, Pyramid,
 // FilterMotion launch flag: the listed pairs are at their finest level, whose duals are not
 // carried into the *_l planes (only a next level's upsampling reads them)
 constexpr int kFilterFinest = 1;
+// LevelInit launch flags: kInitCoarse, the level has a coarser one to upsample; kInitSkipConst,
+// du_new, dv_new, du_acc, dv_acc (0) and dw_acc (= dw) are not written, because the level's
+// first iteration launch is a fresh temporally blocked one, which sets them itself
+constexpr int kInitCoarse = 1, kInitSkipConst = 2;
 
 // Iterations per launch supported by the temporally blocked kernel.
 constexpr int kTBMaxK = 8;
@@ -184,8 +188,9 @@ unsigned batch_blocks_per_level(unsigned pixels);
 unsigned batch_blocks(BatchKernel k, unsigned rows, unsigned cols);
 // blocks of one frame's pyramid level in a batched pyramid launch
 unsigned batch_pyramid_blocks(unsigned level, unsigned rows, unsigned cols);
-// k = 0: whole-level kernel (iters from PDArgs), 1: one iteration, >1: temporal blocking
-hipError_t launch_pd_batch(int k, int shape, const BatchTable &t, unsigned total_blocks, hipStream_t s);
+// k = 0: whole-level kernel (iters from PDArgs), 1: one iteration, >1: temporal blocking;
+// fresh (k > 1 only): the first launch of a level whose level-init ran with kInitSkipConst
+hipError_t launch_pd_batch(int k, int shape, bool fresh, const BatchTable &t, unsigned total_blocks, hipStream_t s);
 unsigned pd_blocks(int k, int shape, unsigned rows, unsigned cols);
 hipError_t launch_motion_field(const LevelArgs &a, hipStream_t s);
 // device-to-device streaming copy (bytes a multiple of 16, 16-byte aligned pointers)

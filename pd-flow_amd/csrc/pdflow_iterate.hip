@@ -201,14 +201,23 @@ __device__ __forceinline__ unsigned px_index(const PDArgs &a, int gv, int gu, bo
 }
 
 // the 13 state planes, ri and rj
+// FRESH: the first launch of a level whose set-up wrote only dw of the primal state: du, dv and
+// their accumulators start at 0 and dw_acc at dw (.cu:447-454, 556-568), set here instead of
+// loaded
+template <bool FRESH = false>
 __device__ __forceinline__ void load_state(Px &q, const PDArgs &a, int gv, int gu, bool on)
 {
     q.gv = gv;
     q.gu = gu;
     q.on = on;
     const unsigned g = px_index(a, gv, gu, on);
-    q.du = gptr(a.in[S_DU])[g]; q.dv = gptr(a.in[S_DV])[g]; q.dw = gptr(a.in[S_DW])[g];
-    q.ua = gptr(a.in[S_DU_ACC])[g]; q.va = gptr(a.in[S_DV_ACC])[g]; q.wa = gptr(a.in[S_DW_ACC])[g];
+    if constexpr (FRESH) {
+        q.dw = gptr(a.in[S_DW])[g];
+        q.du = 0.f; q.dv = 0.f; q.ua = 0.f; q.va = 0.f; q.wa = q.dw;
+    } else {
+        q.du = gptr(a.in[S_DU])[g]; q.dv = gptr(a.in[S_DV])[g]; q.dw = gptr(a.in[S_DW])[g];
+        q.ua = gptr(a.in[S_DU_ACC])[g]; q.va = gptr(a.in[S_DV_ACC])[g]; q.wa = gptr(a.in[S_DW_ACC])[g];
+    }
     q.pd = gptr(a.in[S_PD])[g];
 #pragma unroll
     for (int k = 0; k < 6; k++) q.p[k] = gptr(a.in[S_PUU + k])[g];
@@ -229,9 +238,10 @@ __device__ __forceinline__ void set_consts(Px &q, const float (&c)[kConstPlanes]
     q.ddt = c[5]; q.dct = c[6]; q.dcu = c[7]; q.dcv = c[8];
 }
 
+template <bool FRESH = false>
 __device__ __forceinline__ void load_px(Px &q, const PDArgs &a, int gv, int gu, bool on)
 {
-    load_state(q, a, gv, gu, on);
+    load_state<FRESH>(q, a, gv, gu, on);
     const unsigned g = px_index(a, gv, gu, on);
     float c[kConstPlanes];
 #pragma unroll
@@ -456,7 +466,7 @@ __device__ __forceinline__ void tb_iterate(const PDArgs &a, TBLds<RV, RU> &s, co
     PD_EACH(store);
 }
 
-template <int K, int RV, int RU>
+template <int K, int RV, int RU, bool FRESH = false>
 __device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned by)
 {
     constexpr int TVk = RV - 2 * K, TUk = RU - 2 * K, PX = tb_px<RV, RU>();
@@ -470,7 +480,7 @@ __device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned b
     auto load = [&](Px &q, int j) __attribute__((always_inline)) {
         const int lu = 2 * m.slot + j;
         const int gv = rv0 + m.lv, gu = ru0 + lu;
-        load_px(q, a, gv, gu, gv >= 0 && gv < rows && gu >= 0 && gu < cols);
+        load_px<FRESH>(q, a, gv, gu, gv >= 0 && gv < rows && gu >= 0 && gu < cols);
         s.ri[lu * RV + m.lv] = q.ri;
         s.rj[lu * RV + m.lv] = q.rj;
     };
@@ -572,14 +582,14 @@ __global__ void __launch_bounds__(TV * TU) kb_pd_fused(BatchTable t)
     const unsigned tiles_v = (a.rows + TV - 1) / TV;
     pd_fused_body(a, it.block % tiles_v, it.block / tiles_v);
 }
-template <int K, int RV, int RU>
+template <int K, int RV, int RU, bool FRESH>
 __global__ void __launch_bounds__(kTBThreads) __attribute__((amdgpu_waves_per_eu(tb_waves_per_simd<RV, RU>())))
 kb_pd_tb(BatchTable t)
 {
     const BatchItem it = batch_item(t, xcd_swizzle(blockIdx.x, gridDim.x));
     const PDArgs &a = static_cast<const PDArgs *>(t.args)[it.item];
     const unsigned tiles_v = (a.rows + (RV - 2 * K) - 1) / (RV - 2 * K);
-    tb_body<K, RV, RU>(a, it.block % tiles_v, it.block / tiles_v);
+    tb_body<K, RV, RU, FRESH>(a, it.block % tiles_v, it.block / tiles_v);
 }
 __global__ void __launch_bounds__(BT) kb_pd_level(BatchTable t)
 {
@@ -665,23 +675,29 @@ unsigned pd_blocks(int k, int shape, unsigned rows, unsigned cols)
     return tv * tu;
 }
 
-template <int RV, int RU>
+template <int RV, int RU, bool FRESH>
 static hipError_t launch_tb_batch(int k, const BatchTable &t, const dim3 &grid, hipStream_t s)
 {
     const dim3 block(kTBThreads);
     switch (k) {
-    case 2: hipLaunchKernelGGL((kb_pd_tb<2, RV, RU>), grid, block, 0, s, t); break;
-    case 3: hipLaunchKernelGGL((kb_pd_tb<3, RV, RU>), grid, block, 0, s, t); break;
-    case 4: hipLaunchKernelGGL((kb_pd_tb<4, RV, RU>), grid, block, 0, s, t); break;
-    case 5: hipLaunchKernelGGL((kb_pd_tb<5, RV, RU>), grid, block, 0, s, t); break;
-    case 6: hipLaunchKernelGGL((kb_pd_tb<6, RV, RU>), grid, block, 0, s, t); break;
-    case 8: hipLaunchKernelGGL((kb_pd_tb<8, RV, RU>), grid, block, 0, s, t); break;
+    case 2: hipLaunchKernelGGL((kb_pd_tb<2, RV, RU, FRESH>), grid, block, 0, s, t); break;
+    case 3: hipLaunchKernelGGL((kb_pd_tb<3, RV, RU, FRESH>), grid, block, 0, s, t); break;
+    case 4: hipLaunchKernelGGL((kb_pd_tb<4, RV, RU, FRESH>), grid, block, 0, s, t); break;
+    case 5: hipLaunchKernelGGL((kb_pd_tb<5, RV, RU, FRESH>), grid, block, 0, s, t); break;
+    case 6: hipLaunchKernelGGL((kb_pd_tb<6, RV, RU, FRESH>), grid, block, 0, s, t); break;
+    case 8: hipLaunchKernelGGL((kb_pd_tb<8, RV, RU, FRESH>), grid, block, 0, s, t); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_pd_batch(int k, int shape, const BatchTable &t, unsigned total_blocks, hipStream_t s)
+template <int RV, int RU>
+static hipError_t launch_tb_batch(int k, bool fresh, const BatchTable &t, const dim3 &grid, hipStream_t s)
+{
+    return fresh ? launch_tb_batch<RV, RU, true>(k, t, grid, s) : launch_tb_batch<RV, RU, false>(k, t, grid, s);
+}
+
+hipError_t launch_pd_batch(int k, int shape, bool fresh, const BatchTable &t, unsigned total_blocks, hipStream_t s)
 {
     if (total_blocks == 0 || t.n == 0) return hipSuccess;
     const dim3 grid(total_blocks);
@@ -690,8 +706,8 @@ hipError_t launch_pd_batch(int k, int shape, const BatchTable &t, unsigned total
     case 1: hipLaunchKernelGGL(kb_pd_fused, grid, dim3(TV * TU), 0, s, t); return hipGetLastError();
     default:
         switch (shape) {
-        case kTBShape64x32: return launch_tb_batch<64, 32>(k, t, grid, s);
-        case kTBShape32x32: return launch_tb_batch<32, 32>(k, t, grid, s);
+        case kTBShape64x32: return launch_tb_batch<64, 32>(k, fresh, t, grid, s);
+        case kTBShape32x32: return launch_tb_batch<32, 32>(k, fresh, t, grid, s);
         default: return hipErrorInvalidValue;
         }
     }

@@ -48,7 +48,8 @@ __host__ __device__ inline unsigned tiles_of(unsigned rows, unsigned cols)
 constexpr int kCR = kTR / 2 + 3, kCC = kTC / 2 + 3, kZero = kCR * kCC;
 static_assert(kCR * kCC <= kBlock, "one coarse element per thread and plane");
 
-__device__ __forceinline__ void level_init_body(const LevelArgs &a, unsigned tile, int has_coarse)
+__device__ __forceinline__ void level_init_body(const LevelArgs &a, unsigned tile, int has_coarse,
+                                                bool write_const = true)
 {
     __shared__ float sC[10][kZero + 1];
     const unsigned rows_i = a.rows, cols_i = a.cols;
@@ -133,12 +134,14 @@ __device__ __forceinline__ void level_init_body(const LevelArgs &a, unsigned til
         }
     }
     if (!on) return;
-    // .cu:447-454: du_new, dv_new, du_acc, dv_acc start at 0 on every level
-#pragma unroll
-    for (int k = 0; k < 4; k++) gptr(a.f[kInitZero[k]])[index] = 0.f;
 #pragma unroll
     for (int q = 0; q < 10; q++) gptr(a.f[kUpDst[q]])[index] = acc[q];
-    gptr(a.f[PDF_DW_ACC])[index] = acc[2];
+    if (write_const) {
+        // .cu:447-454: du_new, dv_new, du_acc, dv_acc start at 0 on every level
+#pragma unroll
+        for (int k = 0; k < 4; k++) gptr(a.f[kInitZero[k]])[index] = 0.f;
+        gptr(a.f[PDF_DW_ACC])[index] = acc[2];
+    }
 }
 
 // ------------------------------------------------------------------ warm start
@@ -548,10 +551,11 @@ __global__ void __launch_bounds__(kBlock) k_warp_mu(LevelArgs a, int write_steps
 __global__ void __launch_bounds__(kBlock) k_filter_motion(LevelArgs a) { filter_motion_body(a, blockIdx.x); }
 __global__ void __launch_bounds__(kBlock) k_pyramid2(Pyr2Args a) { pyramid_body<1>(a.frame[blockIdx.y], blockIdx.x); }
 
-__global__ void __launch_bounds__(kBlock) kb_level_init(BatchTable t, int has_coarse)
+__global__ void __launch_bounds__(kBlock) kb_level_init(BatchTable t, int flag)
 {
     const BatchItem it = batch_item(t);
-    level_init_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, has_coarse);
+    level_init_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, flag & kInitCoarse,
+                    (flag & kInitSkipConst) == 0);
 }
 __global__ void __launch_bounds__(kBlock) kb_warm_init(BatchTable t)
 {
