@@ -177,7 +177,9 @@ pdf::LevelArgs level_args(const pdf_batch *b, const Pair &p)
     return a;
 }
 
-pdf::PDArgs pd_args(const pdf_batch *b, const Pair &p, unsigned iters)
+// to_carry: the launch is the level's last and writes its duals into the *_l planes that the next
+// level's upsampling reads, instead of the state planes (which nothing reads afterwards)
+pdf::PDArgs pd_args(const pdf_batch *b, const Pair &p, unsigned iters, bool to_carry = false)
 {
     pdf::PDArgs a;
     a.ri = p.f[PDF_RI]; a.rj = p.f[PDF_RJ];
@@ -187,6 +189,10 @@ pdf::PDArgs pd_args(const pdf_batch *b, const Pair &p, unsigned iters)
     for (int k = 0; k < pdf::S_NUM; k++) {
         a.in[k] = p.f[kState[k]];
         a.out[k] = p.alt[kState[k]];
+    }
+    if (to_carry) {
+        static const int carry[7] = {PDF_PD_L, PDF_PUU_L, PDF_PUV_L, PDF_PVU_L, PDF_PVV_L, PDF_PWU_L, PDF_PWV_L};
+        for (int k = 0; k < 7; k++) a.out[pdf::S_PD + k] = p.f[carry[k]];
     }
     a.rows = p.rows_i; a.cols = p.cols_i;
     a.lambda_i = b->lambda_i; a.lambda_d = b->lambda_d;
@@ -289,10 +295,11 @@ std::vector<std::vector<int>> iteration_chunks(const pdf_batch *b, const std::ve
 }
 
 // fresh[j] (if given): the level-init of active[j] left the constant state planes to its first
-// launch, which is temporally blocked; launches are grouped by size and by that property
+// launch, which is temporally blocked; launches are grouped by size and by that property.
+// to_carry[j] (if given): the last launch of active[j] writes its duals into the *_l planes.
 template <class N>
 void add_iterations(pdf_batch *b, Plan &plan, const std::vector<unsigned> &active, N n_of,
-                    const std::vector<char> &fresh = {})
+                    const std::vector<char> &fresh = {}, const std::vector<char> &to_carry = {})
 {
     const std::vector<std::vector<int>> todo = iteration_chunks(b, active, n_of);
     std::vector<size_t> pos(active.size(), 0);
@@ -314,7 +321,8 @@ void add_iterations(pdf_batch *b, Plan &plan, const std::vector<unsigned> &activ
             if (pos[j] >= todo[j].size() || todo[j][pos[j]] != k || fresh_next(j) != fr) continue;
             Pair &p = b->pairs[active[j]];
             const unsigned n = k == 0 ? n_of(p) : (unsigned)k;
-            args.push_back(pd_args(b, p, n));
+            const bool last = pos[j] + 1 == todo[j].size();
+            args.push_back(pd_args(b, p, n, last && !to_carry.empty() && to_carry[j]));
             blocks.push_back(pdf::pd_blocks(k, b->tb_shape, p.rows_i, p.cols_i));
             pix += (uint64_t)p.rows_i * p.cols_i * n;
             pos[j]++;
@@ -414,17 +422,26 @@ void build_plan(pdf_batch *b, const std::vector<unsigned> &who, bool fallback, P
             fresh[j] = !chunks[j].empty() && chunks[j][0] > 1;
             (fresh[j] ? cold_fresh : cold_full).push_back(active[j]);
         }
+        // below a pair's finest level, a temporally blocked last launch writes the duals where the
+        // next level's upsampling reads them, and the filter then skips that copy
+        std::vector<char> to_carry(active.size(), 0);
+        std::vector<unsigned> finest, inner_copy, inner_direct;
+        for (size_t j = 0; j < active.size(); j++) {
+            const unsigned i = active[j];
+            if (s + 1 == b->pairs[i].ctf_levels) { finest.push_back(i); continue; }
+            to_carry[j] = !chunks[j].empty() && chunks[j].back() > 1;
+            (to_carry[j] ? inner_direct : inner_copy).push_back(i);
+        }
         const int coarse = s > 0 ? pdf::kInitCoarse : 0;
         add_setup(b, plan, pdf::BatchKernel::LevelInit, coarse | pdf::kInitSkipConst, cold_fresh);
         add_setup(b, plan, pdf::BatchKernel::LevelInit, coarse, cold_full);
         add_setup(b, plan, pdf::BatchKernel::WarmInit, 0, warm_now);
         add_setup(b, plan, pdf::BatchKernel::RijGradients, 0, active);
         add_setup(b, plan, pdf::BatchKernel::WarpMu, 0, active);
-        add_iterations(b, plan, active, n_of, fresh);
-        std::vector<unsigned> finest, inner;
-        for (unsigned i : active) (s + 1 == b->pairs[i].ctf_levels ? finest : inner).push_back(i);
-        add_setup(b, plan, pdf::BatchKernel::FilterMotion, 0, inner);
-        add_setup(b, plan, pdf::BatchKernel::FilterMotion, pdf::kFilterFinest, finest);
+        add_iterations(b, plan, active, n_of, fresh, to_carry);
+        add_setup(b, plan, pdf::BatchKernel::FilterMotion, 0, inner_copy);
+        add_setup(b, plan, pdf::BatchKernel::FilterMotion, pdf::kFilterNoCarry, inner_direct);
+        add_setup(b, plan, pdf::BatchKernel::FilterMotion, pdf::kFilterNoCarry, finest);
     }
 }
 
@@ -452,7 +469,7 @@ void build_continue_plan(pdf_batch *b, const std::vector<unsigned> &who, unsigne
             if (b->pairs[i].rows_i * b->pairs[i].cols_i == P) same.push_back(i);
         add_iterations(b, plan, same, [&](const Pair &) { return extra; });
     }
-    add_setup(b, plan, pdf::BatchKernel::FilterMotion, pdf::kFilterFinest, who);
+    add_setup(b, plan, pdf::BatchKernel::FilterMotion, pdf::kFilterNoCarry, who);
 }
 
 void canonicalize(Pair &p)

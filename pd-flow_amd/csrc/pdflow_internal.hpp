@@ -107,9 +107,10 @@ __device__ __forceinline__ BatchItem batch_item(const BatchTable &t, unsigned b)
 __device__ __forceinline__ BatchItem batch_item(const BatchTable &t) { return batch_item(t, blockIdx.x); }
 
 enum class BatchKernel { LevelInit, RijGradients, WarpMu, FilterMotion, Pyramid, WarmInit };
-// FilterMotion launch flag: the listed pairs are at their finest level, whose duals are not
-// carried into the *_l planes (only a next level's upsampling reads them)
-constexpr int kFilterFinest = 1;
+// FilterMotion launch flag: the filter does not copy the duals into the *_l planes, which only a
+// next level's upsampling reads: at a pair's finest level there is none, and at a level whose
+// last iteration launch wrote its duals there directly the copy is done
+constexpr int kFilterNoCarry = 1;
 // LevelInit launch flags: kInitCoarse, the level has a coarser one to upsample; kInitSkipConst,
 // du_new, dv_new, du_acc, dv_acc (0) and dw_acc (= dw) are not written, because the level's
 // first iteration launch is a fresh temporally blocked one, which sets them itself

@@ -572,11 +572,11 @@ __global__ void __launch_bounds__(kBlock) kb_warp_mu(BatchTable t)
     const BatchItem it = batch_item(t);
     warp_mu_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, 0);
 }
-// flag kFilterFinest: the pairs' finest level (no dual carry)
+// flag kFilterNoCarry: the duals are not copied into the *_l planes
 __global__ void __launch_bounds__(kBlock) kb_filter_motion(BatchTable t, int flag)
 {
     const BatchItem it = batch_item(t);
-    filter_motion_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, (flag & kFilterFinest) == 0);
+    filter_motion_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, (flag & kFilterNoCarry) == 0);
 }
 __global__ void __launch_bounds__(kBlock) kb_pyramid(BatchTable t)
 {
