@@ -12,7 +12,11 @@
 // device-resident table of per-pair arguments (BatchTable): per pyramid level one launch for
 // all pairs' two frames; per step four set-up launches and the iteration launches; tiles of all
 // pairs share one grid.  Ping-pong pointer swaps are resolved at plan time, so the sequence is
-// static and is captured once into a HIP graph.
+// static and is captured once into a HIP graph.  Where a level's iterations start and end with
+// temporally blocked launches, the first launch sets the zero state itself (its level-init skips
+// those planes) and the last writes its duals into the carry planes (its filter skips that copy).
+// A launch whose pairs all have the same number of blocks lets each block find its pair by a
+// division instead of a search of the table.
 //
 // Learned warm start: pairs switched to it start at their level of the network's size; after
 // the pyramid the plan gathers those levels into one network input, runs the network's
