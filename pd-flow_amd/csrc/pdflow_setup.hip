@@ -542,6 +542,14 @@ __device__ __forceinline__ void pyramid_body(const PyrArgs &p, unsigned tile)
 }
 
 // ------------------------------------------------------------------ entry points
+// The batched set-up kernels take their blocks in XCD-contiguous order (xcd_swizzle), so that
+// blocks that share halo rows, neighbouring columns or gathered lines run on one XCD and meet
+// in its L2; the blocks of a pair stay consecutive in that order.
+__device__ __forceinline__ BatchItem setup_item(const BatchTable &t)
+{
+    return batch_item(t, xcd_swizzle(blockIdx.x, gridDim.x));
+}
+
 // Single level: arguments by value.  Batched: one launch over many pairs, each block finds
 // its pair's arguments in a device table (see BatchTable).
 __global__ void __launch_bounds__(kBlock) k_level_init(LevelArgs a, int has_coarse) { level_init_body(a, blockIdx.x, has_coarse); }
@@ -553,34 +561,34 @@ __global__ void __launch_bounds__(kBlock) k_pyramid2(Pyr2Args a) { pyramid_body<
 
 __global__ void __launch_bounds__(kBlock) kb_level_init(BatchTable t, int flag)
 {
-    const BatchItem it = batch_item(t);
+    const BatchItem it = setup_item(t);
     level_init_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, flag & kInitCoarse,
                     (flag & kInitSkipConst) == 0);
 }
 __global__ void __launch_bounds__(kBlock) kb_warm_init(BatchTable t)
 {
-    const BatchItem it = batch_item(t);
+    const BatchItem it = setup_item(t);
     warm_init_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block);
 }
 __global__ void __launch_bounds__(kBlock) kb_rij_gradients(BatchTable t)
 {
-    const BatchItem it = batch_item(t);
+    const BatchItem it = setup_item(t);
     rij_gradients_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block);
 }
 __global__ void __launch_bounds__(kBlock) kb_warp_mu(BatchTable t)
 {
-    const BatchItem it = batch_item(t);
+    const BatchItem it = setup_item(t);
     warp_mu_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, 0);
 }
 // flag kFilterNoCarry: the duals are not copied into the *_l planes
 __global__ void __launch_bounds__(kBlock) kb_filter_motion(BatchTable t, int flag)
 {
-    const BatchItem it = batch_item(t);
+    const BatchItem it = setup_item(t);
     filter_motion_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, (flag & kFilterNoCarry) == 0);
 }
 __global__ void __launch_bounds__(kBlock) kb_pyramid(BatchTable t)
 {
-    const BatchItem it = batch_item(t);
+    const BatchItem it = setup_item(t);
     pyramid_body<kPyr0PerThread>(static_cast<const PyrArgs *>(t.args)[it.item], it.block);
 }
 
