@@ -99,7 +99,10 @@ __global__ void __launch_bounds__(kLearnBlock) kb_labels(BatchTable t)
 
 __global__ void __launch_bounds__(kLearnBlock) kb_metrics(BatchTable t, float *partial)
 {
-    const BatchItem it = batch_item(t);
+    // blocks in XCD-contiguous order (neighbouring columns meet in one L2); each block's partial
+    // sums go to its place in that order, so the host's per-pair sums are unchanged
+    const unsigned lb = xcd_swizzle(blockIdx.x, gridDim.x);
+    const BatchItem it = batch_item(t, lb);
     const MetricArgs &a = static_cast<const MetricArgs *>(t.args)[it.item];
     const unsigned rows = a.rows, cols = a.cols;
     const unsigned idx = it.block * kLearnBlock + threadIdx.x;
@@ -133,7 +136,7 @@ __global__ void __launch_bounds__(kLearnBlock) kb_metrics(BatchTable t, float *p
             acc[2] = 1.f;
         }
     }
-    block_sum<3>(acc, partial + 3 * (size_t)blockIdx.x);
+    block_sum<3>(acc, partial + 3 * (size_t)lb);
 }
 
 __global__ void __launch_bounds__(kLearnBlock) kb_epe(BatchTable t, float *partial)
