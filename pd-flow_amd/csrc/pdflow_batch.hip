@@ -1017,7 +1017,7 @@ int pdf_batch_set_learned(pdf_batch *b, pdf_net *net, const unsigned *pairs, uns
 {
     if (!b) return PDF_ERR_ARG;
     drop_plan(b);
-    // switch every learned pair back to a cold start first
+    // switch every learned pair back to a cold start first; on any error below they stay cold
     for (Pair &p : b->pairs)
         if (p.learned) {
             p.learned = false;
@@ -1026,24 +1026,34 @@ int pdf_batch_set_learned(pdf_batch *b, pdf_net *net, const unsigned *pairs, uns
     b->net = nullptr;
     b->net_count = 0;
     if (!net) return PDF_OK;
+    if (count && !pairs) return bfail(b, PDF_ERR_ARG, "null pair list");
     unsigned h, w, maxb;
     pdf::cnn::net_dims(net, &h, &w, &maxb);
     std::vector<unsigned> who;
     if (pairs) who.assign(pairs, pairs + count);
     else for (unsigned i = 0; i < b->pairs.size(); i++) who.push_back(i);
+    // every check before any pair changes
+    std::vector<unsigned> sorted(who);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) return bfail(b, PDF_ERR_ARG, "pair listed twice");
+    if (!sorted.empty() && sorted.back() >= b->pairs.size()) return bfail(b, PDF_ERR_ARG, "bad pair index");
     if (who.size() > maxb) return bfail(b, PDF_ERR_ARG, "more learned pairs than the network's max_batch");
     std::vector<int> level(who.size());
     for (size_t j = 0; j < who.size(); j++) {
-        if (who[j] >= b->pairs.size()) return bfail(b, PDF_ERR_ARG, "bad pair index");
         level[j] = level_of_size(b->pairs[who[j]], h, w);
         if (level[j] < 0)
             return bfail(b, PDF_ERR_ARG, "pair " + std::to_string(who[j]) + " has no level of the network's size");
     }
     if (b->learn_x) { (void)hipFree(b->learn_x); b->learn_x = nullptr; }
-    if (!who.empty()) BHIP(b, hipMalloc((void **)&b->learn_x, who.size() * (size_t)h * w * 4 * sizeof(float)));
+    if (!who.empty()) {
+        const hipError_t e = hipMalloc((void **)&b->learn_x, who.size() * (size_t)h * w * 4 * sizeof(float));
+        if (e != hipSuccess) {
+            b->learn_x = nullptr;
+            return bfail(b, PDF_ERR_NOMEM, std::string("network input planes: ") + hipGetErrorString(e));
+        }
+    }
     for (size_t j = 0; j < who.size(); j++) {
         Pair &p = b->pairs[who[j]];
-        if (p.learned) return bfail(b, PDF_ERR_ARG, "pair listed twice");
         p.learned = true;
         p.start_level = (unsigned)level[j];
         p.net_slot = (unsigned)j;

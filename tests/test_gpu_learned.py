@@ -203,6 +203,33 @@ def test_set_learned_rejects_bad_pairs(pdflow):
     b.set_learned(None)
 
 
+def test_set_learned_error_leaves_every_pair_cold(pdflow):
+    """A rejected pdf_batch_set_learned (a pair listed twice, an index out of range, too many
+    pairs) changes no pair: the next solve is the plain solve, bit for bit."""
+    rows = [240, 240]
+    want = _flows(_plain(pdflow, rows), 2)
+    net = _net(pdflow, 2)
+    b = pdflow.Batch(rows)
+    _fill(pdflow, b, rows)
+    for bad in ([0, 0], [1, 0, 1], [0, 7]):
+        with pytest.raises(pdflow.PDFlowError):
+            b.set_learned(net, bad)
+        b.solve(graph=False)
+        b.synchronize()
+        for i in range(2):
+            for c in range(3):
+                assert np.array_equal(b.flow(i)[c], want[i][c]), (bad, i, c)
+    # a learned set-up followed by a rejected one also leaves the batch plain
+    b.set_learned(net, [1])
+    with pytest.raises(pdflow.PDFlowError):
+        b.set_learned(net, [1, 1])
+    b.solve(graph=True)
+    b.synchronize()
+    for i in range(2):
+        for c in range(3):
+            assert np.array_equal(b.flow(i)[c], want[i][c]), ("after learned", i, c)
+
+
 def test_trained_network_beats_the_ablation(pdflow):
     """The committed network (trained on other seeds) against the same 4x iteration cut without
     it, on held-out QVGA pairs: end-point error against the rendered ground truth."""
