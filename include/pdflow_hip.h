@@ -53,7 +53,7 @@ int pdf_destroy(pdf_ctx *ctx);
 int pdf_set_stream(pdf_ctx *ctx, void *hip_stream);
 int pdf_synchronize(pdf_ctx *ctx);
 const char *pdf_last_error(const pdf_ctx *ctx);
-/* message of the last pdf_create failure in this process */
+/* message of the last pdf_create (or pdf_solve_pair_noalloc) failure on this thread */
 const char *pdf_last_create_error(void);
 
 /* ---- frame ingest and pyramid ------------------------------------------------------ */
@@ -154,6 +154,25 @@ int pdf_solve_graph(pdf_ctx *ctx, const unsigned *num_max_iter);
  * ingest frame 1, pyramid, ingest frame 2, pyramid, solve. */
 int pdf_solve_pair_device(pdf_ctx *ctx, const float *dI1, const float *dZ1, const float *dI2,
                           const float *dZ2, const unsigned *num_max_iter, int use_graph);
+
+/* No counterpart in the reference: pdf_solve_pair_device without a context, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Takes pdf_create's parameters (fovv is unused by every kernel), the
+ * pair's four input planes and three output planes of rows*cols floats (dx, dy, dz: the
+ * finest level's flow), all device memory of the current device, and runs the same kernels as
+ * pdf_solve_pair_device on a default context (same results, bit for bit) on hip_stream (a
+ * hipStream_t of the current device; NULL = the null stream), asynchronously.  It calls no
+ * hipMalloc, hipFree or other allocating routine: its working planes are a device array of
+ * 80 MiB that the library holds on every GPU it runs on until the process ends, enough for the
+ * largest pair the reference takes (640x480 frames); the first call on a device looks up that
+ * array's address.  Every call on one GPU uses the same array, so calls that overlap on one
+ * GPU (on different streams or from different threads) race and must be serialised by the
+ * caller.  Failures return PDF_ERR_* with the message in pdf_last_create_error(). */
+int pdf_solve_pair_noalloc(unsigned rows, unsigned cols, float lambda_i, float lambda_d, float mu,
+                           const float *g_mask25, unsigned ctf_levels, unsigned cam_mode, float fovh,
+                           const float *dI1, const float *dZ1, const float *dI2, const float *dZ2,
+                           const unsigned *num_max_iter, float *dx, float *dy, float *dz,
+                           void *hip_stream);
 
 /* Streaming: the live viewer's loop (main_scene_flow_visualization.cpp:114-130,
  * scene_flow_visualization.cpp:112-208): ingest one new frame from device planes, build its

@@ -42,14 +42,8 @@
 
 namespace {
 
-// planes of the fused solve path (pdflow_setup.hip, pdflow_iterate.hip)
-constexpr int kUsed[] = {PDF_DCT, PDF_DCU, PDF_DCV, PDF_DDT, PDF_DDU, PDF_DDV, PDF_DCU_AUX,
-                         PDF_DCV_AUX, PDF_DDU_AUX, PDF_DDV_AUX, PDF_MU_UV, PDF_RI, PDF_RJ,
-                         PDF_RI_2, PDF_RJ_2, PDF_DU_PREV, PDF_DV_PREV, PDF_DU_NEW, PDF_DV_NEW,
-                         PDF_DW_NEW, PDF_DU_ACC, PDF_DV_ACC, PDF_DW_ACC, PDF_PD, PDF_PUU,
-                         PDF_PUV, PDF_PVU, PDF_PVV, PDF_PWU, PDF_PWV, PDF_DU_L, PDF_DV_L,
-                         PDF_DW_L, PDF_PD_L, PDF_PUU_L, PDF_PUV_L, PDF_PVU_L, PDF_PVV_L,
-                         PDF_PWU_L, PDF_PWV_L, PDF_DX, PDF_DY, PDF_DZ};
+// planes of the fused solve path (pdflow_setup.hip, pdflow_iterate.hip) and the flow
+constexpr int kFlowPlanes[] = {PDF_DX, PDF_DY, PDF_DZ};
 constexpr int kState[pdf::S_NUM] = {PDF_DU_NEW, PDF_DV_NEW, PDF_DW_NEW, PDF_DU_ACC, PDF_DV_ACC,
                                     PDF_DW_ACC, PDF_PD,     PDF_PUU,    PDF_PUV,    PDF_PVU,
                                     PDF_PVV,    PDF_PWU,    PDF_PWV};
@@ -701,7 +695,7 @@ int pdf_batch_create(pdf_batch **out, unsigned npairs, const unsigned *rows, con
         // level 0 of the colour and depth pyramids is the input planes themselves
         bytes += (PDF_NUM_PYR_KINDS - 4) * rnd(P0);
         for (unsigned l = 1; l < p.pyr_levels; l++) bytes += PDF_NUM_PYR_KINDS * rnd(P0 >> (2 * l));
-        bytes += (sizeof(kUsed) / sizeof(kUsed[0]) + pdf::S_NUM) * rnd(P0);
+        bytes += (pdf::kNumFusedPathPlanes + 3 + pdf::S_NUM) * rnd(P0);
         bytes += 3 * rnd(P0);                                   // warm-start planes
     }
     hipError_t e = hipMalloc((void **)&b->arena, bytes);
@@ -725,7 +719,8 @@ int pdf_batch_create(pdf_batch **out, unsigned npairs, const unsigned *rows, con
                 if (l > 0 || (k != PDF_PYR_COLOUR_OLD && k != PDF_PYR_DEPTH_OLD && k != PDF_PYR_COLOUR &&
                               k != PDF_PYR_DEPTH))
                     p.pyr[k][l] = take(P0 >> (2 * l));
-        for (int id : kUsed) p.f[id] = take(P0);
+        for (int id : pdf::kFusedPathPlanes) p.f[id] = take(P0);
+        for (int id : kFlowPlanes) p.f[id] = take(P0);
         for (int k = 0; k < pdf::S_NUM; k++) p.alt[kState[k]] = take(P0);
         p.warm = take(3 * P0);
         p.start_level = 0;

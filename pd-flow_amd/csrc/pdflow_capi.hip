@@ -12,10 +12,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <map>
-#include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/pdflow_hip.h"
@@ -31,7 +32,17 @@ constexpr int kPingPong[] = {PDF_DU_NEW, PDF_DV_NEW, PDF_DW_NEW, PDF_DU_ACC, PDF
 static_assert(sizeof(kPingPong) / sizeof(kPingPong[0]) == pdf::S_NUM, "state plane table");
 constexpr int kNumPingPong = sizeof(kPingPong) / sizeof(kPingPong[0]);
 
-std::string g_create_error;
+// message of the last failure of a call without a context (pdf_create,
+// pdf_solve_pair_noalloc) on this thread
+thread_local char g_create_error[256];
+
+void set_create_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_create_error, sizeof(g_create_error), fmt, ap);
+    va_end(ap);
+}
 
 struct GraphEntry {
     hipGraphExec_t exec = nullptr;
@@ -91,20 +102,21 @@ struct pdf_ctx {
     uint64_t timed_launches = 0, timed_pix = 0;
     double timed_ms = 0.0;
 
-    std::string err;
+    char err[256] = {};
 };
 
 namespace {
 
-int fail(pdf_ctx *c, int code, const std::string &msg)
+int fail(pdf_ctx *c, int code, const char *msg)
 {
-    if (c) c->err = msg;
+    if (c) snprintf(c->err, sizeof(c->err), "%s", msg);
     return code;
 }
 
 int hip_fail(pdf_ctx *c, hipError_t e, const char *what)
 {
-    return fail(c, PDF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    if (c) snprintf(c->err, sizeof(c->err), "%s: %s", what, hipGetErrorString(e));
+    return PDF_ERR_HIP;
 }
 
 #define HIP_TRY(c, call)                                  \
@@ -191,12 +203,95 @@ int finish_timing(pdf_ctx *c)
     return PDF_OK;
 }
 
+struct PairShape {
+    unsigned width, height, pyr_levels;
+};
+
+// pdf_create's argument checks; derives the input frame size and the pyramid depth
+int check_pair_shape(unsigned rows, unsigned cols, unsigned ctf_levels, unsigned cam_mode, PairShape *s,
+                     const char **why)
+{
+    if (cam_mode == 0 || 640 % cam_mode || 480 % cam_mode) {
+        *why = "cam_mode must divide 640 and 480";
+        return PDF_ERR_ARG;
+    }
+    const unsigned width = 640 / cam_mode, height = 480 / cam_mode;
+    if (rows == 0 || cols == 0 || rows > height || cols > width || ctf_levels == 0 ||
+        ctf_levels > PDF_MAX_CTF_LEVELS) {
+        *why = "rows/cols/ctf_levels out of range";
+        return PDF_ERR_ARG;
+    }
+    if ((rows >> (ctf_levels - 1)) << (ctf_levels - 1) != rows ||
+        (cols >> (ctf_levels - 1)) << (ctf_levels - 1) != cols) {
+        *why = "rows and cols must halve exactly ctf_levels-1 times";
+        return PDF_ERR_ARG;
+    }
+    // .cu:41: pyr_levels = round(log2(width/cols)) + ctf_levels
+    const unsigned pyr_levels = (unsigned)roundf(log2f((float)(width / cols))) + ctf_levels;
+    if (pyr_levels > PDF_MAX_PYR_LEVELS) {
+        *why = "too many pyramid levels";
+        return PDF_ERR_ARG;
+    }
+    if ((width >> (pyr_levels - 1)) << (pyr_levels - 1) != width ||
+        (height >> (pyr_levels - 1)) << (pyr_levels - 1) != height) {
+        *why = "input frames must halve exactly for every pyramid level";
+        return PDF_ERR_ARG;
+    }
+    s->width = width;
+    s->height = height;
+    s->pyr_levels = pyr_levels;
+    return PDF_OK;
+}
+
+// readParameters (.cu:176-193): the solver's parameters
+void set_params(pdf_ctx *c, unsigned rows, unsigned cols, float lambda_i, float lambda_d, float mu,
+                const float *g_mask25, unsigned ctf_levels, unsigned cam_mode, float fovh, float fovv,
+                const PairShape &shape)
+{
+    c->rows = rows; c->cols = cols; c->lambda_i = lambda_i; c->lambda_d = lambda_d; c->mu = mu;
+    c->ctf_levels = ctf_levels; c->cam_mode = cam_mode; c->fovh = fovh; c->fovv = fovv;
+    c->width = shape.width; c->height = shape.height; c->pyr_levels = shape.pyr_levels;
+    c->tan_half = tanf(0.5f * fovh);
+    std::memcpy(c->g_mask, g_mask25, sizeof(c->g_mask));
+    c->cap = (size_t)shape.width * shape.height;
+}
+
+// Lays the solver's planes out from base, 256-byte aligned, and returns the bytes they take
+// (base == nullptr: only counts).  full: the ingest planes, every pyramid plane and every field
+// of pdflow_fields.h, as pdf_create holds them; otherwise the pyramids and only the planes of
+// the fused solve path (dx, dy, dz are the caller's), as pdf_solve_pair_noalloc holds them.
+// Both take the 13 alternate state planes of the ping-pong.
+size_t carve_planes(pdf_ctx *c, char *base, bool full)
+{
+    const size_t P0 = (size_t)c->width * c->height;
+    auto rnd = [](size_t floats) { return (floats * sizeof(float) + 255) & ~size_t(255); };
+    size_t off = 0;
+    auto take = [&](size_t floats) {
+        float *r = base ? (float *)(base + off) : nullptr;
+        off += rnd(floats);
+        return r;
+    };
+    if (full) {
+        c->colour_wf = take(P0);
+        c->depth_wf = take(P0);
+    }
+    for (unsigned l = 0; l < c->pyr_levels; l++)
+        for (int k = 0; k < PDF_NUM_PYR_KINDS; k++) c->pyr[k][l] = take(P0 >> (2 * l));
+    if (full) {
+        for (int i = 0; i < PDF_NUM_FIELDS; i++) c->f[i] = take(c->cap);
+    } else {
+        for (int id : pdf::kFusedPathPlanes) c->f[id] = take(c->cap);
+    }
+    for (int k = 0; k < kNumPingPong; k++) c->alt[kPingPong[k]] = take(c->cap);
+    return off;
+}
+
 }  // namespace
 
 extern "C" {
 
-const char *pdf_last_error(const pdf_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
-const char *pdf_last_create_error(void) { return g_create_error.c_str(); }
+const char *pdf_last_error(const pdf_ctx *ctx) { return ctx ? ctx->err : "null context"; }
+const char *pdf_last_create_error(void) { return g_create_error; }
 
 int pdf_device_count(int *count)
 {
@@ -210,67 +305,32 @@ int pdf_create(pdf_ctx **out, unsigned rows, unsigned cols, float lambda_i, floa
                float mu, const float *g_mask25, unsigned ctf_levels, unsigned cam_mode,
                float fovh, float fovv)
 {
-    g_create_error.clear();
-    if (!out || !g_mask25) { g_create_error = "null argument"; return PDF_ERR_ARG; }
+    g_create_error[0] = 0;
+    if (!out || !g_mask25) { set_create_error("null argument"); return PDF_ERR_ARG; }
     *out = nullptr;
-    if (cam_mode == 0 || 640 % cam_mode || 480 % cam_mode) {
-        g_create_error = "cam_mode must divide 640 and 480";
+    PairShape shape;
+    const char *why = nullptr;
+    if (check_pair_shape(rows, cols, ctf_levels, cam_mode, &shape, &why)) {
+        set_create_error("%s", why);
         return PDF_ERR_ARG;
     }
-    const unsigned width = 640 / cam_mode, height = 480 / cam_mode;
-    if (rows == 0 || cols == 0 || rows > height || cols > width || ctf_levels == 0 ||
-        ctf_levels > PDF_MAX_CTF_LEVELS) {
-        g_create_error = "rows/cols/ctf_levels out of range";
-        return PDF_ERR_ARG;
-    }
-    if ((rows >> (ctf_levels - 1)) << (ctf_levels - 1) != rows ||
-        (cols >> (ctf_levels - 1)) << (ctf_levels - 1) != cols) {
-        g_create_error = "rows and cols must halve exactly ctf_levels-1 times";
-        return PDF_ERR_ARG;
-    }
-    // .cu:41: pyr_levels = round(log2(width/cols)) + ctf_levels
-    const unsigned pyr_levels = (unsigned)roundf(log2f((float)(width / cols))) + ctf_levels;
-    if (pyr_levels > PDF_MAX_PYR_LEVELS) { g_create_error = "too many pyramid levels"; return PDF_ERR_ARG; }
-    if ((width >> (pyr_levels - 1)) << (pyr_levels - 1) != width ||
-        (height >> (pyr_levels - 1)) << (pyr_levels - 1) != height) {
-        g_create_error = "input frames must halve exactly for every pyramid level";
-        return PDF_ERR_ARG;
-    }
-
     pdf_ctx *c = new pdf_ctx();
-    c->rows = rows; c->cols = cols; c->lambda_i = lambda_i; c->lambda_d = lambda_d; c->mu = mu;
-    c->ctf_levels = ctf_levels; c->cam_mode = cam_mode; c->fovh = fovh; c->fovv = fovv;
-    c->width = width; c->height = height; c->pyr_levels = pyr_levels;
-    c->tan_half = tanf(0.5f * fovh);
-    std::memcpy(c->g_mask, g_mask25, sizeof(c->g_mask));
-
-    const size_t P0 = (size_t)width * height;
-    c->cap = P0;
-    auto rnd = [](size_t floats) { return (floats * sizeof(float) + 255) & ~size_t(255); };
-    size_t bytes = 2 * rnd(P0);
-    for (unsigned l = 0; l < pyr_levels; l++) bytes += PDF_NUM_PYR_KINDS * rnd(P0 >> (2 * l));
-    bytes += (PDF_NUM_FIELDS + kNumPingPong) * rnd(c->cap);
+    set_params(c, rows, cols, lambda_i, lambda_d, mu, g_mask25, ctf_levels, cam_mode, fovh, fovv, shape);
+    const size_t bytes = carve_planes(c, nullptr, true);
     hipError_t e = hipMalloc((void **)&c->arena, bytes);
     if (e != hipSuccess) {
-        g_create_error = std::string("hipMalloc of the solver arena failed: ") + hipGetErrorString(e);
+        set_create_error("hipMalloc of the solver arena failed: %s", hipGetErrorString(e));
         delete c;
         return PDF_ERR_NOMEM;
     }
     c->arena_bytes = bytes;
-    char *p = c->arena;
-    auto take = [&](size_t floats) { float *r = (float *)p; p += rnd(floats); return r; };
-    c->colour_wf = take(P0);
-    c->depth_wf = take(P0);
-    for (unsigned l = 0; l < pyr_levels; l++)
-        for (int k = 0; k < PDF_NUM_PYR_KINDS; k++) c->pyr[k][l] = take(P0 >> (2 * l));
-    for (int i = 0; i < PDF_NUM_FIELDS; i++) c->f[i] = take(c->cap);
-    for (int k = 0; k < kNumPingPong; k++) c->alt[kPingPong[k]] = take(c->cap);
+    carve_planes(c, c->arena, true);
 
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemsetAsync(c->arena, 0, bytes, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
-        g_create_error = std::string("stream set-up failed: ") + hipGetErrorString(e);
+        set_create_error("stream set-up failed: %s", hipGetErrorString(e));
         if (c->stream) (void)hipStreamDestroy(c->stream);
         (void)hipFree(c->arena);
         delete c;
@@ -491,9 +551,7 @@ int pdf_pd_iterations(pdf_ctx *c, unsigned n)
     if (n == 0) return PDF_OK;
     if (c->level_kernel && pdf::pd_level_supported(c->rows_i * c->cols_i)) return pd_launch_level(c, n);
     const bool tb = c->tb_k > 1 && (size_t)c->rows_i * c->cols_i >= c->tb_min_pixels;
-    for (int k : pdf::tb_chunks(n, tb ? c->tb_k : 1))
-        if ((rc = pd_launch(c, k))) return rc;
-    return PDF_OK;
+    return pdf::for_each_tb_chunk(n, tb ? c->tb_k : 1, [c](int k) { return pd_launch(c, k); });
 }
 
 int pdf_set_iteration_kernel(pdf_ctx *c, int k, unsigned min_pixels)
@@ -849,3 +907,89 @@ int pdf_check_exact_sequence(int which, uint32_t lo, uint32_t hi, uint64_t *mism
 }
 
 }  // extern "C"
+
+// ---- the allocation-free pair solve (pdf_solve_pair_noalloc) --------------------------------
+//
+// Its working planes (both frames' pyramids and the planes of the fused solve path) live in
+// `scratch`, a device array that the code object carries: every GPU the library runs on holds
+// its own copy, from the first use of the library on that GPU until the process ends.  80 MiB
+// covers input frames of 640x480, the largest the reference takes (cam_mode 1): 8 pyramid
+// kinds over 6 levels (13.1 MB) plus 40 fused-path and 13 alternate state planes (65.1 MB).
+// Nothing in it has to start at zero: every plane a solve reads is written earlier in the same
+// solve (the level set-up writes the whole state before the iterations read it).
+namespace {
+constexpr size_t kScratchBytes = size_t(80) << 20;
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+char *g_scratch_ptr[kMaxScratchDevices];
+}  // namespace
+
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+namespace {
+
+// The device address of the current GPU's copy of scratch, looked up once per device with that
+// device current.  In host code the name `scratch` is only a stand-in for the device symbol:
+// kernels and copies get this address, never the name.
+hipError_t scratch_address(char **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<char *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" int pdf_solve_pair_noalloc(unsigned rows, unsigned cols, float lambda_i, float lambda_d, float mu,
+                                      const float *g_mask25, unsigned ctf_levels, unsigned cam_mode, float fovh,
+                                      const float *dI1, const float *dZ1, const float *dI2, const float *dZ2,
+                                      const unsigned *num_max_iter, float *dx, float *dy, float *dz,
+                                      void *hip_stream)
+{
+    g_create_error[0] = 0;
+    if (!g_mask25 || !dI1 || !dZ1 || !dI2 || !dZ2 || !dx || !dy || !dz) {
+        set_create_error("null argument");
+        return PDF_ERR_ARG;
+    }
+    PairShape shape;
+    const char *why = nullptr;
+    if (check_pair_shape(rows, cols, ctf_levels, cam_mode, &shape, &why)) {
+        set_create_error("%s", why);
+        return PDF_ERR_ARG;
+    }
+    // The solver state lives on this stack frame: its containers stay empty (no graphs, no
+    // timing events), so building and destroying it allocates nothing.
+    pdf_ctx c;
+    set_params(&c, rows, cols, lambda_i, lambda_d, mu, g_mask25, ctf_levels, cam_mode, fovh, 0.f, shape);
+    const size_t need = carve_planes(&c, nullptr, false);
+    if (need > kScratchBytes) {
+        set_create_error("frames of %ux%u need %zu bytes of working planes; the allocation-free solve holds %zu",
+                         shape.width, shape.height, need, kScratchBytes);
+        return PDF_ERR_ARG;
+    }
+    char *base = nullptr;
+    const hipError_t e = scratch_address(&base);
+    if (e != hipSuccess) {
+        set_create_error("address of the working planes: %s", hipGetErrorString(e));
+        return PDF_ERR_HIP;
+    }
+    carve_planes(&c, base, false);
+    c.f[PDF_DX] = dx;
+    c.f[PDF_DY] = dy;
+    c.f[PDF_DZ] = dz;
+    c.stream = static_cast<hipStream_t>(hip_stream);
+    c.own_stream = false;
+    const int rc = pdf_solve_pair_device(&c, dI1, dZ1, dI2, dZ2, num_max_iter, 0);
+    if (rc) set_create_error("%s", c.err);
+    return rc;
+}

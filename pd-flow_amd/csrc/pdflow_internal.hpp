@@ -36,6 +36,18 @@ struct Pyr2Args {
     PyrArgs frame[2];
 };
 
+// Planes the fused solve path (k_level_init, k_rij_gradients, k_warp_mu without step planes,
+// the iteration kernels, k_filter_motion) reads or writes, besides dx, dy, dz; the other
+// fields of pdflow_fields.h are written only by the reference-granularity stage kernels.
+constexpr int kFusedPathPlanes[] = {
+    PDF_DCT,     PDF_DCU,     PDF_DCV,     PDF_DDT,     PDF_DDU,    PDF_DDV,    PDF_DCU_AUX,
+    PDF_DCV_AUX, PDF_DDU_AUX, PDF_DDV_AUX, PDF_MU_UV,   PDF_RI,     PDF_RJ,     PDF_RI_2,
+    PDF_RJ_2,    PDF_DU_PREV, PDF_DV_PREV, PDF_DU_NEW,  PDF_DV_NEW, PDF_DW_NEW, PDF_DU_ACC,
+    PDF_DV_ACC,  PDF_DW_ACC,  PDF_PD,      PDF_PUU,     PDF_PUV,    PDF_PVU,    PDF_PVV,
+    PDF_PWU,     PDF_PWV,     PDF_DU_L,    PDF_DV_L,    PDF_DW_L,   PDF_PD_L,   PDF_PUU_L,
+    PDF_PUV_L,   PDF_PVU_L,   PDF_PVV_L,   PDF_PWU_L,   PDF_PWV_L};
+constexpr int kNumFusedPathPlanes = sizeof(kFusedPathPlanes) / sizeof(kFusedPathPlanes[0]);
+
 // State planes of the primal-dual iteration, in the order of PDArgs::in/out.
 enum PDState {
     S_DU = 0, S_DV, S_DW, S_DU_ACC, S_DV_ACC, S_DW_ACC, S_PD,
@@ -125,31 +137,42 @@ bool pd_tb_supported(int k);
 // remainder of 1 joined to the last K) as one supported size or two balanced ones (7 = 4 + 3),
 // so that no level of 2 or more iterations ends with single-iteration launches, which cost
 // about twice a blocked iteration.
-inline std::vector<int> tb_chunks(unsigned n, int K)
+// for_each_tb_chunk calls f(k) for each launch of that split in order, allocates nothing, and
+// stops at the first call that returns nonzero, returning that value (0 if none does).
+template <class F>
+inline int for_each_tb_chunk(unsigned n, int K, F &&f)
 {
-    std::vector<int> out;
     if (K <= 1) {
-        out.assign(n, 1);
-        return out;
+        for (unsigned i = 0; i < n; i++)
+            if (int rc = f(1)) return rc;
+        return 0;
     }
     unsigned q = n / (unsigned)K, r = n % (unsigned)K;
     if (r == 1 && q > 0) {
         q--;
         r += (unsigned)K;
     }
-    for (unsigned i = 0; i < q; i++) out.push_back(K);
-    if (r == 0) return out;
-    if (r == 1 || pd_tb_supported((int)r)) {
-        out.push_back((int)r);
-        return out;
-    }
+    for (unsigned i = 0; i < q; i++)
+        if (int rc = f(K)) return rc;
+    if (r == 0) return 0;
+    if (r == 1 || pd_tb_supported((int)r)) return f((int)r);
     for (unsigned a = (r + 1) / 2; a < r; a++)
         if (pd_tb_supported((int)a) && pd_tb_supported((int)(r - a))) {
-            out.push_back((int)a);
-            out.push_back((int)(r - a));
-            return out;
+            if (int rc = f((int)a)) return rc;
+            return f((int)(r - a));
         }
-    for (unsigned i = 0; i < r; i++) out.push_back(1);
+    for (unsigned i = 0; i < r; i++)
+        if (int rc = f(1)) return rc;
+    return 0;
+}
+
+inline std::vector<int> tb_chunks(unsigned n, int K)
+{
+    std::vector<int> out;
+    for_each_tb_chunk(n, K, [&](int k) {
+        out.push_back(k);
+        return 0;
+    });
     return out;
 }
 

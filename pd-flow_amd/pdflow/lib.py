@@ -85,6 +85,8 @@ def hip_library():
         "pdf_solve": ([vp, up], c_int),
         "pdf_solve_graph": ([vp, up], c_int),
         "pdf_solve_pair_device": ([vp, vp, vp, vp, vp, up, c_int], c_int),
+        "pdf_solve_pair_noalloc": ([c_uint, c_uint, c_float, c_float, c_float, fp, c_uint, c_uint, c_float,
+                                    vp, vp, vp, vp, up, vp, vp, vp, vp], c_int),
         "pdf_get_field": ([vp, c_int, fp], c_int),
         "pdf_set_field": ([vp, c_int, fp], c_int),
         "pdf_get_pyramid": ([vp, c_int, c_uint, fp], c_int),
@@ -368,6 +370,23 @@ class Solver:
         n, ms, pix = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_uint64()
         self._chk(self.lib.pdf_kernel_timing(self.ctx, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(pix)))
         return n.value, ms.value, pix.value
+
+
+def solve_pair_noalloc(bufs, outs, rows=120, cam_mode=4, iters=None, stream=None, **overrides):
+    """One pair through pdf_solve_pair_noalloc: four device buffers in (I1, Z1, I2, Z2), the flow
+    written to three device buffers of rows x cols floats (dx, dy, dz); no device allocation,
+    working planes in the library's per-GPU array.  Asynchronous on `stream` (None: the null
+    stream)."""
+    p = default_params(rows, cam_mode)
+    p.update(overrides)
+    lib = hip_library()
+    mask = np.ascontiguousarray(p["g_mask"], np.float32)
+    arr = None if iters is None else (ctypes.c_uint * len(iters))(*iters)
+    rc = lib.pdf_solve_pair_noalloc(p["rows"], p["cols"], p["lambda_i"], p["lambda_d"], p["mu"], _fptr(mask),
+                                    p["ctf_levels"], p["cam_mode"], p["fovh"],
+                                    *[b.ptr for b in bufs], arr, *[o.ptr for o in outs], stream)
+    if rc:
+        raise PDFlowError(rc, lib.pdf_last_create_error().decode())
 
 
 class Batch:
