@@ -114,6 +114,56 @@ def test_rij_of_fronto_parallel_plane():
     assert np.all(ri[:, -1] == 1.0) and np.all(rj[-1, :] == 1.0)   # zero differences give 1
 
 
+def test_rij_of_two_planes_at_different_depths():
+    """Frame 1 (old) faces the camera at Z1, frame 2 (new) at Z2: ri, rj follow frame 1 and
+    ri_2, rj_2 follow frame 2, each 2/(Z inv_f) from its own frame's point spacing."""
+    o = tiny()
+    rows, cols = o.height, o.width
+    Z1, Z2 = 2.5, 1.25
+    for Zp in (Z1, Z2):
+        o.copy_new_frames(np.zeros(rows * cols, np.float32), np.full(rows * cols, Zp, np.float32))
+        o.gaussian_pyramid()
+    o.stage("rij")
+    inv_f = 2 * np.float64(o.tan_half) / cols
+    for name, Zp in (("ri", Z1), ("rj", Z1), ("ri_2", Z2), ("rj_2", Z2)):
+        r = cm(o.get_field(name), rows)
+        inner = r[:, :-1] if name.startswith("ri") else r[:-1, :]
+        np.testing.assert_allclose(inner, 2 / (Zp * inv_f), rtol=1e-5, err_msg=name)
+
+
+@pytest.mark.parametrize("axis", ["u", "v"])
+def test_image_gradients_follow_the_connected_neighbour(axis):
+    """The central differences are weighted by frame 2's connectivity: across a depth step of
+    frame 2 the difference towards the far side counts less, so the gradient of a parabola at
+    the step lies near the one-sided difference on the connected side.  Frame 1 is flat, so
+    weights taken from it would give the plain mean of the two one-sided differences."""
+    o = tiny()
+    rows, cols = o.height, o.width
+    u = np.repeat(np.arange(cols), rows).astype(np.float64)
+    v = np.tile(np.arange(rows), cols).astype(np.float64)
+    t = u if axis == "u" else v
+    I = (t * t).astype(np.float32)
+    step = 9 if axis == "u" else 6                 # the step lies between t = step and step + 1
+    Z2 = np.where(t > step, 3.5, 2.0).astype(np.float32)
+    o.copy_new_frames(I, np.full(rows * cols, 2.0, np.float32))
+    o.gaussian_pyramid()
+    o.copy_new_frames(I, Z2)
+    o.gaussian_pyramid()
+    o.stage("rij")
+    o.stage("image_gradients")
+    g = cm(o.get_field("dcu_aux" if axis == "u" else "dcv_aux"), rows)
+    back, fwd = 2 * step - 1, 2 * step + 1        # the one-sided differences of t^2 at t = step
+    at = g[:, step] if axis == "u" else g[step, :]
+    assert np.all(at > back) and np.all(at < fwd)                 # a convex combination ...
+    assert np.all(at - back < 0.25 * (fwd - back)), at            # ... weighted to the near side
+    # one pixel beyond the step the far side is the backward one
+    at1 = g[:, step + 1] if axis == "u" else g[step + 1, :]
+    assert np.all((2 * step + 3) - at1 < 0.25 * 2), at1
+    # away from the step both sides are connected equally: the centred difference 2t
+    away = g[:, 3] if axis == "u" else g[3, :]
+    np.testing.assert_allclose(away, 6.0, rtol=1e-6)
+
+
 def test_image_gradients_of_a_ramp():
     o = tiny()
     rows, cols = o.height, o.width
