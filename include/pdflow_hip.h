@@ -49,6 +49,12 @@ int pdf_create(pdf_ctx **out, unsigned rows, unsigned cols, float lambda_i, floa
 /* CSF_cuda::freeDeviceMemory (pdflow_cudalib.cu:249-288). */
 int pdf_destroy(pdf_ctx *ctx);
 
+/* No counterpart in the reference (which never checks an allocation, .cu:36-37): a process-wide
+ * cap on the bytes of one arena (a pdf_create solver's, a pdf_batch_create batch's); 0 = none,
+ * the default.  A create whose arena would exceed it returns PDF_ERR_NOMEM without any device
+ * call, the same status as a failed device allocation.  Arenas that exist are not affected. */
+int pdf_set_arena_limit(size_t bytes);
+
 /* Runs the context's work on a caller-owned hipStream_t (NULL = the library's own). */
 int pdf_set_stream(pdf_ctx *ctx, void *hip_stream);
 int pdf_synchronize(pdf_ctx *ctx);
@@ -224,8 +230,11 @@ int pdf_batch_set_tb_shape(pdf_batch *b, int shape);
  * events) for each region shape and iterations per launch K in {3, 4, 5, 6}, keeps the fastest and
  * returns it (any output may be NULL; *ms = one solve).  Results are identical for every choice.
  * With cache_path, a text file of "<key>\t<shape> <K> <ms>" lines keyed by device and batch
- * composition: a hit applies the cached choice without timing, a miss appends the new line.  The
- * batch's inputs must be set (it is solved several times). */
+ * composition: a hit applies the cached choice without timing, a miss appends the new line; lines
+ * that do not parse to a supported shape and K are skipped.  A cache_path that exists but is not a
+ * readable regular file (a directory, no read permission) applies the default (32 x 32, K = 4)
+ * without timing and returns PDF_OK with *ms = 0.  The batch's inputs must be set (it is solved
+ * several times). */
 int pdf_batch_autotune(pdf_batch *b, const char *cache_path, int *shape, int *k, float *ms);
 /* device pointers of pair's four input planes (the caller may fill them directly).  They are
  * also level 0 of the pair's colour and depth pyramids, so new inputs take effect there at once:

@@ -24,6 +24,8 @@
 // du_prev, dv_prev, dw.  A fallback re-solves chosen pairs in place with the plain path (level
 // 0, the reference schedule) from the pyramids already built.
 #include <hip/hip_runtime.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <array>
@@ -91,6 +93,9 @@ struct Plan {
 
 }  // namespace
 
+// the library's default iteration kernel for batches (the winner of the sweeps on 256 VGA pairs)
+constexpr int kDefaultTBShape = pdf::kTBShape32x32, kDefaultTBK = 4;
+
 struct pdf_batch {
     float lambda_i = 0, lambda_d = 0, mu = 0, tan_half = 0;
     float g_mask[25] = {};
@@ -99,8 +104,8 @@ struct pdf_batch {
     size_t arena_bytes = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    int tb_k = 4;
-    int tb_shape = pdf::kTBShape32x32;
+    int tb_k = kDefaultTBK;
+    int tb_shape = kDefaultTBShape;
     unsigned tb_min_pixels = 60 * 80;
     bool level_kernel = true;
     // plan
@@ -698,6 +703,13 @@ int pdf_batch_create(pdf_batch **out, unsigned npairs, const unsigned *rows, con
         bytes += (pdf::kNumFusedPathPlanes + 3 + pdf::S_NUM) * rnd(P0);
         bytes += 3 * rnd(P0);                                   // warm-start planes
     }
+    const size_t limit = pdf::arena_limit();
+    if (limit && bytes > limit) {
+        g_batch_create_error = "the batch arena needs " + std::to_string(bytes) + " bytes, above the limit of " +
+                               std::to_string(limit) + " (pdf_set_arena_limit)";
+        delete b;
+        return PDF_ERR_NOMEM;
+    }
     hipError_t e = hipMalloc((void **)&b->arena, bytes);
     if (e != hipSuccess) {
         g_batch_create_error = std::string("hipMalloc of the batch arena failed: ") + hipGetErrorString(e);
@@ -795,6 +807,19 @@ int pdf_batch_autotune(pdf_batch *b, const char *cache_path, int *shape_out, int
     const std::string key = tune_key(b);
     std::vector<std::string> lines;
     if (cache_path) {
+        // A cache path that exists but is no readable regular file (a directory, no permission)
+        // is a broken deployment, not a first run: take the library's default tile without
+        // timing, so that every start does not pay for a search it cannot record.
+        struct stat st;
+        if (::stat(cache_path, &st) == 0 && (!S_ISREG(st.st_mode) || ::access(cache_path, R_OK) != 0)) {
+            int rc = pdf_batch_set_tb_shape(b, kDefaultTBShape);
+            if (!rc) rc = pdf_batch_set_iteration_kernel(b, kDefaultTBK, b->tb_min_pixels, b->level_kernel ? 1 : 0);
+            if (rc) return rc;
+            if (shape_out) *shape_out = kDefaultTBShape;
+            if (k_out) *k_out = kDefaultTBK;
+            if (ms_out) *ms_out = 0.f;
+            return PDF_OK;
+        }
         std::ifstream in(cache_path);
         std::string line;
         while (std::getline(in, line)) {

@@ -11,6 +11,7 @@
 //   - every call reports a status instead of ignoring cudaError_t (.cu:36-37).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -21,6 +22,13 @@
 
 #include "../../include/pdflow_hip.h"
 #include "pdflow_internal.hpp"
+
+namespace pdf {
+namespace {
+std::atomic<size_t> g_arena_limit{0};
+}
+size_t arena_limit() { return g_arena_limit.load(std::memory_order_relaxed); }
+}  // namespace pdf
 
 namespace {
 
@@ -317,6 +325,12 @@ int pdf_create(pdf_ctx **out, unsigned rows, unsigned cols, float lambda_i, floa
     pdf_ctx *c = new pdf_ctx();
     set_params(c, rows, cols, lambda_i, lambda_d, mu, g_mask25, ctf_levels, cam_mode, fovh, fovv, shape);
     const size_t bytes = carve_planes(c, nullptr, true);
+    const size_t limit = pdf::arena_limit();
+    if (limit && bytes > limit) {
+        set_create_error("the solver arena needs %zu bytes, above the limit of %zu (pdf_set_arena_limit)", bytes, limit);
+        delete c;
+        return PDF_ERR_NOMEM;
+    }
     hipError_t e = hipMalloc((void **)&c->arena, bytes);
     if (e != hipSuccess) {
         set_create_error("hipMalloc of the solver arena failed: %s", hipGetErrorString(e));
@@ -338,6 +352,12 @@ int pdf_create(pdf_ctx **out, unsigned rows, unsigned cols, float lambda_i, floa
     }
     c->own_stream = true;
     *out = c;
+    return PDF_OK;
+}
+
+int pdf_set_arena_limit(size_t bytes)
+{
+    pdf::g_arena_limit.store(bytes, std::memory_order_relaxed);
     return PDF_OK;
 }
 

@@ -176,6 +176,10 @@ inline std::vector<int> tb_chunks(unsigned n, int K)
     return out;
 }
 
+// Process-wide cap on the bytes one arena (a solver's, a batch's) may take, 0 = none; set by
+// pdf_set_arena_limit and checked on the host before any device allocation.
+size_t arena_limit();
+
 hipError_t launch_pyramid(const PyrArgs &a, hipStream_t s);
 hipError_t launch_assign_zeros(const LevelArgs &a, hipStream_t s);
 hipError_t launch_upsample(const LevelArgs &a, hipStream_t s);

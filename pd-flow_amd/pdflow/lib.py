@@ -67,6 +67,7 @@ def hip_library():
         "pdf_create": ([ctypes.POINTER(vp), c_uint, c_uint, c_float, c_float, c_float, fp, c_uint,
                         c_uint, c_float, c_float], c_int),
         "pdf_destroy": ([vp], c_int),
+        "pdf_set_arena_limit": ([ctypes.c_size_t], c_int),
         "pdf_set_stream": ([vp, vp], c_int),
         "pdf_synchronize": ([vp], c_int),
         "pdf_last_error": ([vp], ctypes.c_char_p),

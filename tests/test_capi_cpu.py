@@ -105,3 +105,29 @@ def test_field_table_matches_header():
     body = text[text.index("enum pdf_field_id"):text.index("PDF_NUM_FIELDS")]
     ids = re.findall(r"PDF_([A-Z0-9_]+)", body)
     assert [x.lower() for x in ids] == list(plib.FIELDS.keys())
+
+
+def test_arena_limit_fails_creates_with_nomem_before_touching_the_device():
+    """Allocator-failure injection from the host: with an arena cap below what a solver or a batch
+    needs, pdf_create and pdf_batch_create return PDF_ERR_NOMEM (4) before any device call, so this
+    runs without a GPU.  Lifting the cap restores the normal path (no GPU here: argument errors
+    still come first, and the allocation itself is not attempted by this test)."""
+    lib = plib.hip_library()
+    mask = pdflow.gaussian_mask()
+    fp = mask.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    ctx = ctypes.c_void_p()
+    try:
+        assert lib.pdf_set_arena_limit(1 << 20) == 0                 # 1 MiB
+        rc = lib.pdf_create(ctypes.byref(ctx), 120, 160, 0.04, 0.35, 75.0, fp, 4, 4, 1.0, 0.8)
+        assert rc == 4 and not ctx.value
+        msg = lib.pdf_last_create_error().decode()
+        assert "limit" in msg and "1048576" in msg, msg
+        rows = (ctypes.c_uint * 2)(240, 120)
+        b = ctypes.c_void_p()
+        rc = lib.pdf_batch_create(ctypes.byref(b), 2, rows, None, 0.04, 0.35, 75.0, fp, 1.0, None)
+        assert rc == 4 and not b.value
+        assert "limit" in lib.pdf_batch_last_error(None).decode()
+        # a bad argument is still reported as such under the cap
+        assert lib.pdf_create(ctypes.byref(ctx), 120, 160, 0.04, 0.35, 75.0, fp, 4, 7, 1.0, 0.8) == 1
+    finally:
+        lib.pdf_set_arena_limit(0)

@@ -172,6 +172,39 @@ def test_autotune_picks_a_config_and_caches_it(gpu_available, tmp_path):
     assert len(cache.read_text().splitlines()) == 1
 
 
+def test_autotune_falls_back_to_the_default_tile(gpu_available, tmp_path):
+    """A cache path that is not a readable file gives the library default (32x32, K = 4) at once,
+    without timing; a cache line that does not parse is skipped and the batch is tuned as usual.
+    The flow is the same either way."""
+    rows = [120, 120]
+    say("autotune fallback on a 2-pair batch")
+    b = Batch(rows)
+    want = []
+    for i, r in enumerate(rows):
+        inputs, flow = single(r, 950 + i)
+        b.set_inputs(i, *inputs)
+        want.append(flow)
+    b.set_tb_shape(0)
+    b.set_iteration_kernel(6, 60 * 80)
+    not_a_file = tmp_path / "cache_dir"
+    not_a_file.mkdir()
+    assert b.autotune(not_a_file) == (1, 4, 0.0)
+    b.solve(graph=True)
+    b.synchronize()
+    for i in range(len(rows)):
+        for a, w in zip(b.flow(i), want[i]):
+            np.testing.assert_array_equal(a, w)
+    # a corrupt line (unsupported shape and K) under this batch's key is not applied
+    good = tmp_path / "probe.txt"
+    b.autotune(good)
+    key = good.read_text().splitlines()[0].split("\t")[0]
+    bad = tmp_path / "bad.txt"
+    bad.write_text(f"{key}\t7 13 0.5\n{key}\tgarbage\n")
+    shape, k, ms = b.autotune(bad)
+    assert shape in (0, 1) and k in (3, 4, 5, 6) and ms > 0
+    assert bad.read_text().splitlines()[-1].split("\t")[1].split()[:2] == [str(shape), str(k)]
+
+
 def test_full_size_batch_matches_single_pairs(gpu_available):
     """256 VGA pairs in one batch, the benchmark's size (a 141557760-thread iteration grid, about
     12 GB of planes): sampled pairs, the last one included, equal their single-pair solves."""
