@@ -1,13 +1,15 @@
 """The command-line pair evaluator (pd-flow_amd/pdflow/pdflow-impair, the counterpart of the
 reference's Scene-Flow-Impair app) end to end: PNG inputs -> solve on the GPU -> the results
-text file and the RGB representation, against the single-pair solver."""
+text file and the RGB representation, against the CPU oracle's solve of the same decoded frames
+(scene_flow_impair.cpp:218-277 input, :95-168 solve, :280-350 output)."""
 import os
 import subprocess
 
 import numpy as np
 import pytest
 
-from pdflow import Solver, png, synth_pair
+from oracle import Oracle
+from pdflow import png, synth_pair
 
 pytestmark = pytest.mark.gpu
 
@@ -25,7 +27,7 @@ def _write_inputs(d, seed, H, W):
 
 
 @pytest.mark.parametrize("rows,H,W", [(240, 240, 320), (120, 480, 640)])
-def test_cli_matches_the_solver(gpu_available, tmp_path, rows, H, W):
+def test_cli_matches_the_oracle(gpu_available, tmp_path, rows, H, W):
     paths = _write_inputs(tmp_path, 77 + rows, H, W)
     root = str(tmp_path / "out")
     cmd = [CLI, "--rows", str(rows), "--i1", paths["i1"], "--i2", paths["i2"], "--z1", paths["z1"],
@@ -34,15 +36,16 @@ def test_cli_matches_the_solver(gpu_available, tmp_path, rows, H, W):
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     cols = rows * 4 // 3
-    # the same frames through the single-pair solver (I = u8, Z = u16 / 5000, as loaded)
+    # the same frames, decoded by the PNG reader (I = u8, Z = u16 / 5000, as the reference
+    # loads them), through the oracle
     I1, Z1 = png.load_frame(paths["i1"], paths["z1"], W, H)
     I2, Z2 = png.load_frame(paths["i2"], paths["z2"], W, H)
-    s = Solver(rows, 480 // H)
-    dx, dy, dz = s.solve_pair(I1, Z1, I2, Z2)
+    dx, dy, dz = Oracle(rows, 480 // H, threads=8).solve_pair(I1, Z1, I2, Z2)
     lines = open(root + "_results01.txt").read().splitlines()
     assert len(lines) == rows * cols
-    # "v u dx dy dz", v-major (scene_flow_impair.cpp:331-339), std::ostream default formatting
-    for k in range(0, rows * cols, 97):
+    # "v u dx dy dz", v-major (scene_flow_impair.cpp:331-339), std::ostream default formatting;
+    # every line, so every pixel's flow equals the oracle's to the printed digits
+    for k in range(rows * cols):
         v, u = k // cols, k % cols
         i = v + u * rows
         want = f"{v} {u} {float(dx[i]):g} {float(dy[i]):g} {float(dz[i]):g}"
