@@ -1,19 +1,23 @@
 """Benchmark of the PD-Flow scene-flow path on one MI355X.
 
 Default workload (--workload learned): 256 VGA (480x640) RGB-D pairs per step, batched with the
-learned warm start -- both pyramids of every pair, one network pass over their 60x80 levels,
-the solves from there with every level's iterations divided by --factor (2.5), up to three stages of
-16 more finest-level iterations (--continue-iters) for the pairs that miss the rule, then the rule (final
-energy or warped residual more than 5% above the plain solver's) and the plain re-solve of the
-pairs it rejects, all inside the timed region; two batches are in flight, so that one batch's
-continuation and fallback overlap the next batch's solve (--pipeline).  The line also carries the plain single-pair and
-plain batched rates of the same pairs, end-point errors against the plain solve and against the
-rendered ground truth, and the fallback fraction.
+learned warm start -- both pyramids of every pair, one network pass over their 60x80 levels, the
+solves from there with the calibrated schedule (every level's iterations divided by 2.5, the finest
+level's budget chosen on training pairs, pdflow/calibrate.py), then the acceptance rule computed from
+the learned solve alone (finest-level energy per pixel, warped residual against the zero-motion
+residual, bounds calibrated on the plain solver's training results), up to three stages of 16 more
+finest-level iterations for the pairs outside the bounds, and the plain re-solve of those still
+outside -- all inside the timed region, with two batches in flight so that one batch's rule,
+continuation and fallback overlap the next batch's solve (--pipeline).  The line also carries the
+rate of the rule as the benchmark configuration words it (the plain solve of every pair inside the
+step, its energy and residual as the references), the plain single-pair and plain batched rates of
+the same pairs, end-point errors against the plain solve and the rendered ground truth, and how
+often the calibrated rule agrees with the plain-referenced one.
 
 --workload plain: the drop-in path on one VGA pair per step (reference schedule 25..100).
 Inputs are seeded synthetic pairs already resident in HBM when the timed region starts.  One
-JSON line is printed, including the primal-dual iteration launches' roofline (148 algorithmic
-bytes per pixel-iteration over their HIP-event-timed launches) and a CPU baseline.
+JSON line is printed, including the roofline of the primal-dual iteration launches (148 bytes per
+pixel per launch over their HIP-event-timed duration) and a CPU baseline.
 """
 import argparse
 import json
@@ -28,15 +32,60 @@ import numpy as np  # noqa: E402
 
 METRIC = "VGA scene-flow pairs/s: batched+learned-init vs plain, same EPE; HBM % peak"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_PIXEL_ITER = 148     # 11 constant planes read + 13 state planes read and written, FP32
-# HBM bytes per temporally blocked launch (kb_pd_tb<K, 32, 32>) over the learned solve's launch mix
-# at 256 VGA pairs, by iteration cut factor, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes
-# of this workload, scaled by the counters' readings for a plain copy of known size (fetch x 2.0,
-# write x 1.0); the passes and the per-level figures are in
-# profiles/pd_iteration_batched_learned_vga256_hbm_traffic.txt.  The 2.5 entry includes the fresh
-# first launches of 120x160, 240x320 and 480x640 (76 B/px read instead of 96); the 2.0 and 4.0
-# entries count every launch at 96 B/px read, so they are about 1.5% high
-TRAFFIC_TB_LEARNED_VGA256 = {2.0: 4.789e9, 2.5: 4.593e9, 4.0: 4.885e9}
+# One primal-dual iteration launch (any number of iterations k) must read the 11 per-level
+# constant planes and the 13 state planes and write the 13 state planes once per pixel: 37 FP32
+# words = 148 bytes per pixel per launch, the algorithmic HBM bytes of the roofline.  Dividing the
+# same bytes by pixel-iterations instead gives an "effective" rate that temporal blocking lifts
+# above the HBM peak; it is reported under its own name.
+BYTES_PER_PIXEL_LAUNCH = 148
+PROFILES = os.path.join(ROOT, "profiles")
+ITERATE_SRC = os.path.join(ROOT, "pd-flow_amd", "csrc", "pdflow_iterate.hip")
+RULE_JSON = os.path.join(ROOT, "pd-flow_amd", "pdflow", "weights", "warmstart_qvga_rule.json")
+
+
+def source_sha16(path):
+    import hashlib
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
+
+def measured_traffic(workload_key):
+    """HBM bytes per launch of the iteration kernel for this workload, from the committed
+    rocprofv3 --pmc summary (profiles/*_hbm_traffic.txt): its 'traffic:' line must name this
+    workload and the iteration kernels' current source hash, else the reading is stale and None
+    is returned with the reason"""
+    sha = source_sha16(ITERATE_SRC)
+    if not os.path.isdir(PROFILES):
+        return None, "no profiles/ directory"
+    for name in sorted(os.listdir(PROFILES)):
+        if not name.endswith("_hbm_traffic.txt"):
+            continue
+        with open(os.path.join(PROFILES, name)) as f:
+            for line in f:
+                if not line.startswith("traffic:"):
+                    continue
+                kv = dict(t.split("=", 1) for t in line[len("traffic:"):].split())
+                if kv.get("workload") == workload_key:
+                    if kv.get("iterate_source_sha16") != sha:
+                        return None, f"{name} was measured on another build of the iteration kernels"
+                    return float(kv["bytes_per_launch"]), f"profiles/{name}"
+    return None, "no traffic profile of this workload"
+
+
+def roofline(kernel, timing, workload_key=None):
+    """roofline of the dominant kernel from HIP-event timing (launches, ms, pixel-iterations,
+    pixels covered) on the stream the launches run on"""
+    launches, kms, pix_iters, pixels = timing
+    sec = kms * 1e-3
+    achieved = BYTES_PER_PIXEL_LAUNCH * pixels / sec / 1e9
+    traffic, source = measured_traffic(workload_key) if workload_key else (None, "not profiled")
+    return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": source,
+            "algorithmic_bytes_per_launch": round(BYTES_PER_PIXEL_LAUNCH * pixels / launches),
+            "avg_launch_us": round(kms / launches * 1e3, 3), "launches": launches,
+            "bytes_per_pixel_per_launch": BYTES_PER_PIXEL_LAUNCH,
+            "effective_gbs_over_pixel_iterations": round(BYTES_PER_PIXEL_LAUNCH * pix_iters / sec / 1e9, 1)}
 
 
 def log(msg):
@@ -99,12 +148,10 @@ def main():
                     help="plain: one pair per step; batched: --batch-pairs pairs of --rows per "
                          "step; mixed: 128 pairs (50%% 480x640, 30%% 240x320, 20%% 120x160); "
                          "learned: --batch-pairs pairs of --rows per step, network warm start at "
-                         "60x80, iterations / --factor, fallback rule")
+                         "60x80, calibrated schedule and acceptance rule (--rule)")
     ap.add_argument("--batch-pairs", type=int, default=256)
-    ap.add_argument("--factor", type=float, default=2.5,
-                    help="learned: every level's iteration count divided by this (2.5 with three "
-                         "continuation stages gives the highest throughput once continuation and "
-                         "fallback re-solves are counted)")
+    ap.add_argument("--rule", default=RULE_JSON,
+                    help="learned: the calibrated schedule and acceptance bounds (pdflow/calibrate.py)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "pd-flow_amd", "pdflow", "weights", "warmstart_qvga.bin"))
     ap.add_argument("--autotune", default=None, metavar="CACHE",
                     help="batched/learned: pick the kernel configuration with pdf_batch_autotune "
@@ -114,20 +161,16 @@ def main():
                          "of HBM byte counters under rocprofv3 --pmc)")
     ap.add_argument("--schedule", default=None, metavar="N,N,...",
                     help="learned: the whole per-level iteration schedule (coarse to fine) in place of "
-                         "the cut one")
-    ap.add_argument("--finest-iters", type=int, default=None, metavar="N",
-                    help="learned: the finest level's iteration count in place of the cut schedule's")
+                         "the calibrated one")
     ap.add_argument("--continue-iters", default="16,16,16", metavar="N[,N...]",
-                    help="learned: pairs that miss the fallback rule first get N more finest-level "
+                    help="learned: pairs outside the rule's bounds first get N more finest-level "
                          "iterations from where they stand (pdf_batch_continue), stage by stage for "
                          "a list, each stage re-checking the rule; the plain path then re-solves "
-                         "those that still miss it (0: no continuation)")
+                         "those still outside (0: no continuation)")
     ap.add_argument("--pipeline", type=int, choices=(0, 1), default=1,
                     help="learned: 1 = two batches in flight on their own streams, so that one batch's "
                          "continuation and fallback overlap the next batch's solve; 0 = one batch, "
                          "steps back to back")
-    ap.add_argument("--fallback", choices=("energy_residual", "residual", "none"), default="energy_residual",
-                    help="learned: rule deciding which pairs the plain path re-solves (tolerance 5%%)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -206,11 +249,10 @@ def main():
         rc = lib.pdf_solve_pair_device(ctx, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr, arr, 0)
         if rc:
             raise RuntimeError(lib.pdf_last_error(ctx).decode())
-    launches, kms, pix = s.kernel_timing()
+    timing = s.kernel_timing()
     s.enable_kernel_timing(False)
-    achieved = BYTES_PER_PIXEL_ITER * pix / (kms * 1e-3) / 1e9
-    log(f"bench: fused PD kernel {launches} launches, avg {kms / launches * 1e3:.2f} us, "
-        f"{achieved:.0f} GB/s algorithmic")
+    rl = roofline("primal-dual iteration launches (k_pd_tb<K>, k_pd_level, k_pd_fused)", timing)
+    log(f"bench: iteration launches {timing[0]}, avg {rl['avg_launch_us']:.2f} us, {rl['achieved']:.0f} GB/s")
 
     # flow sanity: the solve produced finite values
     dx, dy, dz = s.motion_field()
@@ -235,18 +277,7 @@ def main():
             "rows": rows, "cols": rows * 4 // 3, "cam_mode": cam_mode, "pairs_per_step": 1,
             "pixel_iterations_per_pair": int(sum(r * c * n for (_, r, c, _, n) in table)),
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "primal-dual iteration launches (k_pd_tb<K>, k_pd_level, k_pd_fused)",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "avg_launch_us": round(kms / launches * 1e3, 3),
-            "launches": launches,
-            "bytes_per_pixel_iteration": BYTES_PER_PIXEL_ITER,
-        },
+        "roofline": rl,
     }
     if not args.no_cpu_baseline:
         log("bench: CPU baseline ...")
@@ -332,10 +363,10 @@ def run_batched(args):
     log(f"bench: {args.steps} batched solves in {el:.3f} s -> {pairs_s:.1f} pairs/s")
     b.enable_kernel_timing(True)
     b.solve(graph=False)
-    launches, kms, pix = b.kernel_timing()
+    timing = b.kernel_timing()
     b.enable_kernel_timing(False)
-    achieved = BYTES_PER_PIXEL_ITER * pix / (kms * 1e-3) / 1e9
-    log(f"bench: PD launches {launches}, {kms:.3f} ms, {achieved:.0f} GB/s algorithmic")
+    rl = roofline("kb_pd_tb / kb_pd_level (batched primal-dual iterations)", timing)
+    log(f"bench: PD launches {timing[0]}, {timing[1]:.3f} ms, {rl['achieved']:.0f} GB/s")
     for i in (0, len(rows_list) - 1):
         for a in b.flow(i):
             assert np.isfinite(a).all()
@@ -356,10 +387,7 @@ def run_batched(args):
         "config": {"workload": name + (", hipGraph" if not args.no_graph else ""),
                    "pairs_per_step": len(rows_list),
                    "pixel_iterations_per_step": int(sum(pix_pair[r] for r in rows_list))},
-        "roofline": {"bound": "hbm", "kernel": "kb_pd_tb / kb_pd_level (batched primal-dual iterations)",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "launches": launches, "bytes_per_pixel_iteration": BYTES_PER_PIXEL_ITER},
+        "roofline": rl,
     }
     print(json.dumps(out), flush=True)
 
@@ -388,26 +416,27 @@ def plain_single_pair_rate(rows, seed, steps=20):
 
 def run_learned(args):
     """Batched + learned warm start: one step = one batched solve of all pairs (pyramids, one
-    network pass over their 60x80 levels, injection, solves with iterations / factor), then the
-    fallback rule and the plain re-solve of the pairs it rejects."""
+    network pass over their 60x80 levels, injection, solves with the calibrated schedule), then the
+    calibrated acceptance rule, continuation of the pairs outside it and the plain re-solve of
+    those still outside."""
     from pdflow import Batch, Net, synth_pairs_float
-    from pdflow.train import cut_schedule
     rows, n = args.rows, args.batch_pairs
     cam = 480 // rows
-    if not os.path.exists(args.weights):
-        raise SystemExit(f"bench.py: network weights {args.weights} are missing")
+    for path in (args.weights, args.rule):
+        if not os.path.exists(path):
+            raise SystemExit(f"bench.py: {path} is missing")
+    with open(args.rule) as f:
+        rule = json.load(f)
     log(f"bench: rendering {n} pairs of {480 // cam}x{640 // cam}")
     *planes, gt = synth_pairs_float(args.seed, n, 480 // cam, 640 // cam, with_gt=True)
     net = Net(max_batch=n, h=60, w=80)
     net.load(args.weights)
     stages = [int(x) for x in str(args.continue_iters).split(",") if int(x) > 0]
-    cut = cut_schedule(rows, args.factor)
-    if args.schedule:
-        cut = [int(x) for x in args.schedule.split(",")]
-        if len(cut) != len(cut_schedule(rows, args.factor)):
-            raise SystemExit(f"bench.py: --schedule needs {len(cut_schedule(rows, args.factor))} levels")
-    if args.finest_iters is not None:
-        cut[-1] = args.finest_iters
+    sched = [int(x) for x in args.schedule.split(",")] if args.schedule else list(rule["schedule"])
+    if len(sched) != 6 and rows == 480:
+        raise SystemExit("bench.py: --schedule needs 6 levels")
+    emax, rmax = rule["energy_per_pixel_max"], rule["residual_ratio_max"]
+    P = rows * (rows * 4 // 3)
 
     def make(iters=None):
         b = Batch([rows] * n, iters=None if iters is None else [iters] * n)
@@ -417,12 +446,7 @@ def run_learned(args):
             b.set_inputs(i, *(p[i] for p in planes))
         return b
 
-    # references of the fallback rule and of the EPE: the plain solve of the same pairs (untimed)
-    plain = make()
-    plain.solve(graph=True)
-    plain.synchronize()
-    pe, pr = plain.metrics()
-    learned = make(cut)
+    learned = make(sched)
     learned.set_learned(net)
     if args.autotune:
         log(f"bench: autotuned (shape, K, ms) = {learned.autotune(args.autotune)}")
@@ -432,40 +456,36 @@ def run_learned(args):
     if args.pipeline:
         net2 = Net(max_batch=n, h=60, w=80)
         net2.load(args.weights)
-        learned2 = make(cut)
+        learned2 = make(sched)
         learned2.set_learned(net2)
         if args.autotune:
             learned2.autotune(args.autotune)
         batches.append(learned2)
-    ablation = make(cut)
-    inf = np.full(n, np.inf, np.float32)
-    e_ref = {"energy_residual": pe, "residual": inf, "none": inf}[args.fallback]
-    r_ref = {"energy_residual": pr, "residual": pr, "none": inf}[args.fallback]
 
-    def misses(e, r, idx):
-        return ~(e <= 1.05 * e_ref[idx]) | ~(r <= 1.05 * r_ref[idx])
+    def outside(e, r, r0):
+        return ~(e <= emax * P) | ~(r <= rmax * r0)
 
     def finish(b):
-        """the rule after the learned solve of batch b: optional continuation of the pairs that
-        miss it (re-checked alone), then the plain re-solve of those that still do; returns
-        (continued, fell back) flags"""
-        e, r = b.metrics()
-        bad = np.nonzero(misses(e, r, slice(None)))[0]
+        """the calibrated rule after the learned solve of batch b: pairs outside the bounds get the
+        continuation stages (re-checked alone after each), the plain path re-solves those still
+        outside; returns (continued, fell back) flags"""
+        e, r, r0 = b.quality()
+        bad = np.nonzero(outside(e, r, r0))[0]
         cont = np.zeros(n, np.uint8)
         for extra in stages:
             if not len(bad):
                 break
             cont[bad] = 1
             b.continue_finest(bad, extra)
-            e2, r2 = b.metrics_pairs(bad)
-            bad = bad[misses(e2, r2, bad)]
+            e2, r2, r02 = b.quality(bad)
+            bad = bad[outside(e2, r2, r02)]
         b.resolve_plain(bad)
         flags = np.zeros(n, np.uint8)
         flags[bad] = 1
         return cont, flags
 
-    def run_steps(k):
-        """k steps, each a whole batch (solve, continuation, fallback).  Pipelined: the next
+    def run_steps(k, batches, finish):
+        """k steps, each a whole batch (solve, rule, continuation, fallback).  Pipelined: the next
         batch's solve is launched on its own stream before this batch's finish, whose host
         synchronisations wait on this batch's stream only."""
         graph = not args.no_graph
@@ -483,13 +503,43 @@ def run_learned(args):
             b.synchronize()
         return flags
 
-    flags = run_steps(max(1, args.warmup))
+    flags = run_steps(max(1, args.warmup), batches, finish)
     t0 = time.perf_counter()
-    flags = run_steps(args.steps)
+    flags = run_steps(args.steps, batches, finish)
     el = time.perf_counter() - t0
     pairs_s = n * args.steps / el
     log(f"bench: {args.steps} learned steps in {el:.3f} s -> {pairs_s:.1f} pairs/s "
         f"(fallback {int(flags.sum())}/{n})")
+
+    # the rule as the configuration words it: every step also solves every pair with the plain
+    # path, whose energy and residual are the references; pairs more than 5% above them get the
+    # same continuation stages, then the plain re-solve
+    plain = make()
+    steps_ref = max(1, args.steps // 2)
+
+    def finish_ref(b):
+        plain.solve(graph=not args.no_graph)
+        pe, pr, _ = plain.quality()
+        e, r, _ = b.quality()
+        bad = np.nonzero(~(e <= 1.05 * pe) | ~(r <= 1.05 * pr))[0]
+        for extra in stages:
+            if not len(bad):
+                break
+            b.continue_finest(bad, extra)
+            e2, r2, _ = b.quality(bad)
+            bad = bad[~(e2 <= 1.05 * pe[bad]) | ~(r2 <= 1.05 * pr[bad])]
+        b.resolve_plain(bad)
+        flags = np.zeros(n, np.uint8)
+        flags[bad] = 1
+        return None, flags
+
+    run_steps(1, [learned], finish_ref)
+    t0 = time.perf_counter()
+    run_steps(steps_ref, [learned], finish_ref)
+    plain.synchronize()
+    rule_as_worded = n * steps_ref / (time.perf_counter() - t0)
+    log(f"bench: rule as worded (plain solves inside the step) {rule_as_worded:.1f} pairs/s")
+
     # the learned solve alone (no rule), and the plain batched solve, for the same pairs
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -504,7 +554,9 @@ def run_learned(args):
     plain_single = plain_single_pair_rate(rows, args.seed)
     log(f"bench: learned alone {learned_only:.1f}, plain batched {plain_batched:.1f}, "
         f"plain single pair {plain_single:.1f} pairs/s")
-    # quality: EPE against the plain solve and against the rendered ground truth
+
+    # quality on these pairs: end-point errors against the plain solve and the rendered truth, and
+    # how the calibrated rule's decisions compare with the plain-referenced rule's
     def gt_epe(b):
         out = []
         for i in range(n):
@@ -513,17 +565,23 @@ def run_learned(args):
             out.append(d.mean())
         return np.asarray(out)
 
+    pe, pr, _ = plain.quality()
     learned.solve(graph=True)
     learned.synchronize()
+    le, lr_, l0 = learned.quality()
+    self_ok = ~outside(le, lr_, l0)
+    plain_ok = (le <= 1.05 * pe) & (lr_ <= 1.05 * pr)
     epe_l = learned.epe(plain)
-    le, lr_ = learned.metrics()
     gt_learned = gt_epe(learned)
     cont, flags = finish(learned)
     epe_f = learned.epe(plain)
+    gt_plain, gt_final = gt_epe(plain), gt_epe(learned)
+    ablation = make(sched)
     ablation.solve(graph=True)
     ablation.synchronize()
     epe_a = ablation.epe(plain)
-    gt_plain, gt_final, gt_abl = gt_epe(plain), gt_epe(learned), gt_epe(ablation)
+    gt_abl = gt_epe(ablation)
+    del ablation
 
     def stats(a):
         return {"mean": round(float(np.mean(a)), 6), "median": round(float(np.median(a)), 6),
@@ -534,9 +592,11 @@ def run_learned(args):
     learned.enable_kernel_timing(True, tb_only=True)
     for _ in range(3):
         learned.solve(graph=False)
-    launches, kms, pix = learned.kernel_timing()
+    timing = learned.kernel_timing()
     learned.enable_kernel_timing(False)
-    achieved = BYTES_PER_PIXEL_ITER * pix / (kms * 1e-3) / 1e9
+    key = f"learned-{n}x{rows}-{','.join(map(str, sched))}-shape{1 if args.tb_shape is None else args.tb_shape}"
+    rl = roofline("kb_pd_tb<K, 32, 32> (batched, K = 2 to 5 primal-dual iterations per launch)", timing, key)
+    ho = rule.get("heldout", {})
     out = {
         "metric": METRIC,
         "value": round(pairs_s, 3),
@@ -550,40 +610,40 @@ def run_learned(args):
         "vs_baseline": None,
         "dtype": "f32 (network bf16 x bf16 -> f32)",
         "data": f"synthetic (seeded RGB-D generator, seeds {args.seed}..{args.seed + n - 1})",
-        "config": {"workload": f"batched + learned warm start, {n} pairs of {rows}x{rows * 4 // 3}: "
-                               f"network at the 60x80 level, iterations {cut} "
-                               + ("(given), " if args.schedule else f"(reference / {args.factor}"
-                                  + (f", finest level {args.finest_iters}" if args.finest_iters is not None else "") + "), ")
-                               + (f"pairs missing the rule continued by {'+'.join(map(str, stages))} finest-level "
-                                  "iterations, " if stages else "")
-                               + f"fallback rule '{args.fallback}' (5%) with plain re-solves inside the timed "
+        "config": {"workload": f"batched + learned warm start, {n} pairs of {rows}x{rows * 4 // 3}: network at "
+                               f"the 60x80 level, iterations {sched} (reference / {rule['factor']}, finest level "
+                               f"calibrated), acceptance rule computed from the learned solve alone (finest-level "
+                               f"energy <= {emax:.3f} per pixel and warped residual <= {rmax:.4f} x the zero-motion "
+                               f"residual, bounds from the plain solver's training results), pairs outside it "
+                               + (f"continued by {'+'.join(map(str, stages))} finest-level iterations, then "
+                                  if stages else "")
+                               + "re-solved by the plain path; rule, continuation and re-solves inside the timed "
                                "region" + (", hipGraph" if not args.no_graph else "")
-                               + (", two batches in flight (one's continuation and fallback overlap the "
-                                  "next one's solve)" if args.pipeline else ""),
+                               + (", two batches in flight" if args.pipeline else ""),
                    "pairs_per_step": n, "batches_in_flight": len(batches),
-                   "weights": os.path.relpath(args.weights, ROOT)},
+                   "weights": os.path.relpath(args.weights, ROOT), "rule": os.path.relpath(args.rule, ROOT)},
         "vs_plain": {"plain_single_pair_pairs_s": round(plain_single, 3),
                      "plain_batched_pairs_s": round(plain_batched, 3),
-                     "learned_without_fallback_pairs_s": round(learned_only, 3),
+                     "learned_without_rule_pairs_s": round(learned_only, 3),
+                     "rule_as_worded_pairs_s": round(rule_as_worded, 3),
                      "speedup_vs_plain_single_pair": round(pairs_s / plain_single, 3),
-                     "speedup_without_fallback": round(learned_only / plain_single, 3)},
-        "quality_m": {"epe_vs_plain_learned": stats(epe_l), "epe_vs_plain_after_fallback": stats(epe_f),
+                     "speedup_vs_plain_batched": round(pairs_s / plain_batched, 3)},
+        "rule": {"accepted_fraction": round(float(self_ok.mean()), 4),
+                 "plain_referenced_rule_accepts": round(float(plain_ok.mean()), 4),
+                 "agreement_with_plain_referenced_rule": round(float((self_ok == plain_ok).mean()), 4),
+                 "accepted_but_plain_rule_rejects": round(float((self_ok & ~plain_ok).mean()), 4),
+                 "heldout_calibration_check": {k: ho.get(k) for k in ("pairs", "agree", "rule_accepts",
+                                                                      "plain_rule_accepts",
+                                                                      "accepted_by_rule_rejected_by_plain_rule",
+                                                                      "corrupted_network")},
+                 "continued_fraction": round(float(cont.mean()), 4), "continue_iters": stages,
+                 "fallback_fraction": round(float(flags.mean()), 4)},
+        "quality_m": {"epe_vs_plain_learned": stats(epe_l), "epe_vs_plain_final": stats(epe_f),
                       "epe_vs_plain_ablation": stats(epe_a), "epe_vs_truth_plain": stats(gt_plain),
-                      "epe_vs_truth_learned": stats(gt_learned), "epe_vs_truth_after_fallback": stats(gt_final),
+                      "epe_vs_truth_learned": stats(gt_learned), "epe_vs_truth_final": stats(gt_final),
                       "epe_vs_truth_ablation": stats(gt_abl),
-                      "energy_ratio_learned": stats(le / pe), "residual_ratio_learned": stats(lr_ / pr),
-                      "continued_fraction": round(float(cont.mean()), 4),
-                      "continue_iters": stages,
-                      "fallback_fraction": round(float(flags.mean()), 4)},
-        "roofline": {"bound": "hbm", "kernel": "kb_pd_tb<K, 32, 32> (batched, K = 2 to 5 primal-dual iterations per launch)",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": (TRAFFIC_TB_LEARNED_VGA256.get(float(args.factor))
-                                 if (rows, n) == (480, 256) else None),
-                     "traffic_algorithmic_bytes": round(BYTES_PER_PIXEL_ITER * pix / launches),
-                     "launches": launches,
-                     "avg_launch_us": round(kms / launches * 1e3, 3),
-                     "bytes_per_pixel_iteration": BYTES_PER_PIXEL_ITER},
+                      "energy_ratio_to_plain": stats(le / pe), "residual_ratio_to_plain": stats(lr_ / pr)},
+        "roofline": rl,
     }
     if not args.no_cpu_baseline:
         log("bench: CPU baseline ...")

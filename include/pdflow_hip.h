@@ -200,10 +200,12 @@ float pdf_tan_half_fovh(const pdf_ctx *ctx);
 
 /* Kernel timing of the fused iteration: when enabled, every pdf_pd_iterations launch
  * is bracketed by HIP events on the context's stream; pdf_kernel_timing returns the
- * number of timed launches, their total time (ms) and the pixel-iterations they did. */
+ * number of timed launches, their total time (ms), the pixel-iterations they did and the pixels
+ * of the levels they covered (a launch of k iterations over P pixels counts k*P and P: each
+ * launch streams its level's planes through HBM once, whatever k).  Outputs may be NULL. */
 int pdf_enable_kernel_timing(pdf_ctx *ctx, int enable);
 int pdf_kernel_timing(pdf_ctx *ctx, uint64_t *launches, double *total_ms,
-                      uint64_t *pixel_iters);
+                      uint64_t *pixel_iters, uint64_t *pixels);
 
 /* ---- batched ragged solve (no reference counterpart: the reference solves one pair per
  * CSF_cuda object) -------------------------------------------------------------------- */
@@ -267,7 +269,8 @@ int pdf_batch_launch_count(const pdf_batch *b, unsigned *launches);
 /* HIP-event timing of the iteration launches, as pdf_enable_kernel_timing (solves run
  * eagerly while enabled); enable = 2 times only the temporally blocked launches */
 int pdf_batch_enable_kernel_timing(pdf_batch *b, int enable);
-int pdf_batch_kernel_timing(pdf_batch *b, uint64_t *launches, double *total_ms, uint64_t *pixel_iters);
+int pdf_batch_kernel_timing(pdf_batch *b, uint64_t *launches, double *total_ms, uint64_t *pixel_iters,
+                            uint64_t *pixels);
 /* planes of pair's last solve on the device: a field of include/pdflow_fields.h (those the
  * batched path keeps, at the finest level; the dual carry planes pd_l and p*_l hold the last
  * coarser level's duals, since nothing reads them after the finest level) or a pyramid plane
@@ -297,6 +300,20 @@ int pdf_batch_metrics(pdf_batch *b, float *energy, float *residual);
 /* No counterpart in the reference: pdf_batch_metrics for the listed pairs only; energy[j] and
  * residual[j] belong to pairs[j] (for re-checking the pairs a continuation touched) */
 int pdf_batch_metrics_pairs(pdf_batch *b, const unsigned *pairs, unsigned count, float *energy, float *residual);
+/* No counterpart in the reference: per listed pair (all pairs if pairs is NULL, in pair order),
+ * pdf_batch_metrics' energy and residual and the zero-motion residual, mean |I2(x) - I1(x)| over
+ * the same pixels with frame-1 depth: the intensity change a flow has to explain.  Any output may
+ * be NULL. */
+int pdf_batch_quality(pdf_batch *b, const unsigned *pairs, unsigned count, float *energy, float *residual,
+                      float *residual0);
+/* No counterpart in the reference: the learned path's acceptance rule that needs no plain solve.
+ * Every pair whose finest-level energy exceeds energy_per_pixel_max times its pixel count, or whose
+ * residual exceeds residual_ratio_max times its zero-motion residual (pdf_batch_quality), is
+ * re-solved by the plain path (pdf_batch_resolve_plain); NaN values count as exceeding.  The bounds
+ * are calibrated on the plain solver's results over training pairs (pdflow/calibrate.py).
+ * flags[i] = 1 for the re-solved pairs (flags may be NULL), *count their number. */
+int pdf_batch_fallback_bounds(pdf_batch *b, float energy_per_pixel_max, float residual_ratio_max,
+                              unsigned char *flags, unsigned *count);
 /* per pair: mean 3-D end-point error (metres) between this batch's flow and ref's over pixels
  * with frame-1 depth (pairs of equal sizes) */
 int pdf_batch_epe(pdf_batch *b, const pdf_batch *ref, float *epe);

@@ -81,6 +81,7 @@ struct Launch {
     unsigned n, blocks;
     unsigned stride;        // blocks per item when every item has the same number, else 0
     uint64_t pixel_iters;   // PD launches: pixel-iterations the launch performs
+    uint64_t pixels;        // PD launches: pixels of the levels the launch covers
 };
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -119,8 +120,8 @@ struct pdf_batch {
     bool timing = false;
     bool timing_tb_only = false;     // time only the temporally blocked iteration launches
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
-    std::vector<uint64_t> event_pix;
-    uint64_t timed_launches = 0, timed_pix = 0;
+    std::vector<std::pair<uint64_t, uint64_t>> event_pix;   // pixel-iterations, pixels
+    uint64_t timed_launches = 0, timed_pix = 0, timed_pixl = 0;
     double timed_ms = 0.0;
     // learned warm start
     pdf_net *net = nullptr;
@@ -319,7 +320,7 @@ void add_iterations(pdf_batch *b, Plan &plan, const std::vector<unsigned> &activ
         if (k < 0) break;
         std::vector<pdf::PDArgs> args;
         std::vector<unsigned> blocks;
-        uint64_t pix = 0;
+        uint64_t pix = 0, pixl = 0;
         for (size_t j = 0; j < active.size(); j++) {
             if (pos[j] >= todo[j].size() || todo[j][pos[j]] != k || fresh_next(j) != fr) continue;
             Pair &p = b->pairs[active[j]];
@@ -328,6 +329,7 @@ void add_iterations(pdf_batch *b, Plan &plan, const std::vector<unsigned> &activ
             args.push_back(pd_args(b, p, n, last && !to_carry.empty() && to_carry[j]));
             blocks.push_back(pdf::pd_blocks(k, b->tb_shape, p.rows_i, p.cols_i));
             pix += (uint64_t)p.rows_i * p.cols_i * n;
+            pixl += (uint64_t)p.rows_i * p.cols_i;
             pos[j]++;
             swap_state(p);
         }
@@ -336,6 +338,7 @@ void add_iterations(pdf_batch *b, Plan &plan, const std::vector<unsigned> &activ
         l.k = k;
         l.flag = fr ? 1 : 0;
         l.pixel_iters = pix;
+        l.pixels = pixl;
         add_launch(plan, l, args, blocks);
     }
 }
@@ -556,7 +559,7 @@ int run_plan(pdf_batch *b, const Plan &plan, char *dev_tables, bool timed)
         if (ev) {
             BHIP(b, hipEventRecord(e1, b->stream));
             b->events.emplace_back(e0, e1);
-            b->event_pix.push_back(l.pixel_iters);
+            b->event_pix.emplace_back(l.pixel_iters, l.pixels);
         }
     }
     return PDF_OK;
@@ -570,7 +573,8 @@ int finish_timing(pdf_batch *b)
         float ms = 0.f;
         BHIP(b, hipEventElapsedTime(&ms, b->events[i].first, b->events[i].second));
         b->timed_ms += ms;
-        b->timed_pix += b->event_pix[i];
+        b->timed_pix += b->event_pix[i].first;
+        b->timed_pixl += b->event_pix[i].second;
         b->timed_launches++;
         (void)hipEventDestroy(b->events[i].first);
         (void)hipEventDestroy(b->events[i].second);
@@ -985,11 +989,13 @@ int pdf_batch_enable_kernel_timing(pdf_batch *b, int enable)
     b->timing_tb_only = enable == 2;
     b->timed_launches = 0;
     b->timed_pix = 0;
+    b->timed_pixl = 0;
     b->timed_ms = 0.0;
     return PDF_OK;
 }
 
-int pdf_batch_kernel_timing(pdf_batch *b, uint64_t *launches, double *total_ms, uint64_t *pixel_iters)
+int pdf_batch_kernel_timing(pdf_batch *b, uint64_t *launches, double *total_ms, uint64_t *pixel_iters,
+                            uint64_t *pixels)
 {
     if (!b) return PDF_ERR_ARG;
     int rc = pdf_batch_synchronize(b);
@@ -997,6 +1003,7 @@ int pdf_batch_kernel_timing(pdf_batch *b, uint64_t *launches, double *total_ms, 
     if (launches) *launches = b->timed_launches;
     if (total_ms) *total_ms = b->timed_ms;
     if (pixel_iters) *pixel_iters = b->timed_pix;
+    if (pixels) *pixels = b->timed_pixl;
     return PDF_OK;
 }
 
@@ -1122,7 +1129,8 @@ int pdf_batch_continue(pdf_batch *b, const unsigned *pairs, unsigned count, unsi
 namespace {
 
 // per listed pair: the finest level's linearised energy and warped-intensity residual
-int metrics_of(pdf_batch *b, const std::vector<unsigned> &who, float *energy, float *residual)
+int metrics_of(pdf_batch *b, const std::vector<unsigned> &who, float *energy, float *residual,
+               float *residual0 = nullptr)
 {
     Plan m;
     std::vector<pdf::MetricArgs> args;
@@ -1150,24 +1158,27 @@ int metrics_of(pdf_batch *b, const std::vector<unsigned> &who, float *energy, fl
     add_launch(m, l, args, blocks);
     const Launch &L = m.launches[0];
     char *tables = nullptr, *extra = nullptr;
-    int rc = upload_scratch_plan(b, m, 3 * sizeof(float) * L.blocks, &tables, &extra);
+    constexpr int NP = pdf::kMetricPartials;
+    int rc = upload_scratch_plan(b, m, NP * sizeof(float) * L.blocks, &tables, &extra);
     if (rc) return rc;
     pdf::BatchTable t{tables + L.args_off, (const unsigned *)(tables + L.first_off), L.n, L.stride};
     hipError_t e = pdf::launch_learn(L.learn, t, L.blocks, (float *)extra, b->stream);
     if (e != hipSuccess) return bfail(b, PDF_ERR_HIP, std::string("metrics launch: ") + hipGetErrorString(e));
-    std::vector<float> part(3 * (size_t)L.blocks);
+    std::vector<float> part(NP * (size_t)L.blocks);
     BHIP(b, hipMemcpyAsync(part.data(), extra, part.size() * sizeof(float), hipMemcpyDeviceToHost, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
     size_t blk = 0;
     for (size_t j = 0; j < who.size(); j++) {
-        double es = 0, rs = 0, rc_ = 0;
+        double es = 0, rs = 0, rc_ = 0, r0 = 0;
         for (unsigned q = 0; q < blocks[j]; q++, blk++) {
-            es += part[3 * blk];
-            rs += part[3 * blk + 1];
-            rc_ += part[3 * blk + 2];
+            es += part[NP * blk];
+            rs += part[NP * blk + 1];
+            rc_ += part[NP * blk + 2];
+            r0 += part[NP * blk + 3];
         }
         if (energy) energy[j] = (float)es;
         if (residual) residual[j] = (float)(rc_ > 0 ? rs / rc_ : 0.0);
+        if (residual0) residual0[j] = (float)(rc_ > 0 ? r0 / rc_ : 0.0);
     }
     return PDF_OK;
 }
@@ -1191,6 +1202,44 @@ int pdf_batch_metrics_pairs(pdf_batch *b, const unsigned *pairs, unsigned count,
     for (unsigned i : who)
         if (i >= b->pairs.size()) return bfail(b, PDF_ERR_ARG, "bad pair index");
     return metrics_of(b, who, energy, residual);
+}
+
+int pdf_batch_quality(pdf_batch *b, const unsigned *pairs, unsigned count, float *energy, float *residual,
+                      float *residual0)
+{
+    if (!b || (count && !pairs)) return PDF_ERR_ARG;
+    if (!b->planned) return bfail(b, PDF_ERR_STATE, "quality before any solve");
+    std::vector<unsigned> who;
+    if (pairs) {
+        who.assign(pairs, pairs + count);
+        for (unsigned i : who)
+            if (i >= b->pairs.size()) return bfail(b, PDF_ERR_ARG, "bad pair index");
+    } else {
+        for (unsigned i = 0; i < b->pairs.size(); i++) who.push_back(i);
+    }
+    return metrics_of(b, who, energy, residual, residual0);
+}
+
+int pdf_batch_fallback_bounds(pdf_batch *b, float energy_per_pixel_max, float residual_ratio_max,
+                              unsigned char *flags, unsigned *count)
+{
+    if (!b || !(energy_per_pixel_max > 0.f) || !(residual_ratio_max > 0.f)) return PDF_ERR_ARG;
+    const size_t n = b->pairs.size();
+    std::vector<float> e(n), r(n), r0(n);
+    std::vector<unsigned> all(n);
+    for (unsigned i = 0; i < n; i++) all[i] = i;
+    if (!b->planned) return bfail(b, PDF_ERR_STATE, "fallback before any solve");
+    int rc = metrics_of(b, all, e.data(), r.data(), r0.data());
+    if (rc) return rc;
+    std::vector<unsigned> who;
+    for (size_t i = 0; i < n; i++) {
+        const float pixels = (float)b->pairs[i].rows * (float)b->pairs[i].cols;
+        const bool bad = !(e[i] <= energy_per_pixel_max * pixels) || !(r[i] <= residual_ratio_max * r0[i]);
+        if (flags) flags[i] = bad ? 1 : 0;
+        if (bad) who.push_back((unsigned)i);
+    }
+    if (count) *count = (unsigned)who.size();
+    return pdf_batch_resolve_plain(b, who.data(), (unsigned)who.size());
 }
 
 int pdf_batch_epe(pdf_batch *b, const pdf_batch *ref, float *epe)

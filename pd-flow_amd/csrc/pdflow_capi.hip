@@ -106,8 +106,8 @@ struct pdf_ctx {
 
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
-    std::vector<uint64_t> event_pix;
-    uint64_t timed_launches = 0, timed_pix = 0;
+    std::vector<std::pair<uint64_t, uint64_t>> event_pix;   // pixel-iterations, pixels
+    uint64_t timed_launches = 0, timed_pix = 0, timed_pixl = 0;
     double timed_ms = 0.0;
 
     char err[256] = {};
@@ -201,7 +201,8 @@ int finish_timing(pdf_ctx *c)
         float ms = 0.f;
         HIP_TRY(c, hipEventElapsedTime(&ms, c->events[i].first, c->events[i].second));
         c->timed_ms += ms;
-        c->timed_pix += c->event_pix[i];
+        c->timed_pix += c->event_pix[i].first;
+        c->timed_pixl += c->event_pix[i].second;
         c->timed_launches++;
         (void)hipEventDestroy(c->events[i].first);
         (void)hipEventDestroy(c->events[i].second);
@@ -538,7 +539,7 @@ static int pd_launch(pdf_ctx *c, int k)
     if (c->timing) {
         HIP_TRY(c, hipEventRecord(e1, c->stream));
         c->events.emplace_back(e0, e1);
-        c->event_pix.push_back((uint64_t)c->rows_i * c->cols_i * (uint64_t)k);
+        c->event_pix.emplace_back((uint64_t)c->rows_i * c->cols_i * (uint64_t)k, (uint64_t)c->rows_i * c->cols_i);
     }
     swap_pingpong(c);
     return PDF_OK;
@@ -558,7 +559,7 @@ static int pd_launch_level(pdf_ctx *c, unsigned n)
     if (c->timing) {
         HIP_TRY(c, hipEventRecord(e1, c->stream));
         c->events.emplace_back(e0, e1);
-        c->event_pix.push_back((uint64_t)c->rows_i * c->cols_i * (uint64_t)n);
+        c->event_pix.emplace_back((uint64_t)c->rows_i * c->cols_i * (uint64_t)n, (uint64_t)c->rows_i * c->cols_i);
     }
     swap_pingpong(c);
     return PDF_OK;
@@ -866,11 +867,12 @@ int pdf_enable_kernel_timing(pdf_ctx *c, int enable)
     c->timing = enable != 0;
     c->timed_launches = 0;
     c->timed_pix = 0;
+    c->timed_pixl = 0;
     c->timed_ms = 0.0;
     return PDF_OK;
 }
 
-int pdf_kernel_timing(pdf_ctx *c, uint64_t *launches, double *total_ms, uint64_t *pixel_iters)
+int pdf_kernel_timing(pdf_ctx *c, uint64_t *launches, double *total_ms, uint64_t *pixel_iters, uint64_t *pixels)
 {
     if (!c) return PDF_ERR_ARG;
     int rc = pdf_synchronize(c);
@@ -878,6 +880,7 @@ int pdf_kernel_timing(pdf_ctx *c, uint64_t *launches, double *total_ms, uint64_t
     if (launches) *launches = c->timed_launches;
     if (total_ms) *total_ms = c->timed_ms;
     if (pixel_iters) *pixel_iters = c->timed_pix;
+    if (pixels) *pixels = c->timed_pixl;
     return PDF_OK;
 }
 

@@ -13,7 +13,8 @@
 //    the L1 data terms of the prox (pdflow_cudalib.cu:906-923) and the depth dual
 //    (.cu:836-846), the TV terms of the duals (.cu:849-885) with grad_r the r-weighted forward
 //    differences of .cu:982-1013, and the warped-intensity residual |I2(x + u) - I1(x)| over
-//    pixels with frame-1 depth (bilinear warp of .cu:654-682);
+//    pixels with frame-1 depth (bilinear warp of .cu:654-682), and the zero-motion residual
+//    |I2(x) - I1(x)| over the same pixels (the intensity change the flow has to explain);
 //  - Epe: the 3-D end-point error between two flows over pixels with frame-1 depth.
 // Reductions: wave64 butterfly (__shfl_xor) then the block's four waves through LDS; one
 // partial per block, summed per pair on the host in double precision.
@@ -106,7 +107,7 @@ __global__ void __launch_bounds__(kLearnBlock) kb_metrics(BatchTable t, float *p
     const MetricArgs &a = static_cast<const MetricArgs *>(t.args)[it.item];
     const unsigned rows = a.rows, cols = a.cols;
     const unsigned idx = it.block * kLearnBlock + threadIdx.x;
-    float acc[3] = {0.f, 0.f, 0.f};
+    float acc[kMetricPartials] = {0.f, 0.f, 0.f, 0.f};
     if (idx < rows * cols) {
         const unsigned v = idx % rows, u = idx / rows;
         const float du = gptr(a.du)[idx], dv = gptr(a.dv)[idx], dw = gptr(a.dw)[idx];
@@ -132,11 +133,13 @@ __global__ void __launch_bounds__(kLearnBlock) kb_metrics(BatchTable t, float *p
         acc[0] = data_i + data_d + tv;
         if (gptr(a.Zold)[idx] > 0.f) {
             const Tap tp = make_tap(rows, cols, (float)u + gptr(a.du_l)[idx], (float)v + gptr(a.dv_l)[idx]);
-            acc[1] = fabsf(bilinear(a.I2, rows, tp) - gptr(a.I1)[idx]);
+            const float i1 = gptr(a.I1)[idx];
+            acc[1] = fabsf(bilinear(a.I2, rows, tp) - i1);
             acc[2] = 1.f;
+            acc[3] = fabsf(gptr(a.I2)[idx] - i1);
         }
     }
-    block_sum<3>(acc, partial + 3 * (size_t)lb);
+    block_sum<kMetricPartials>(acc, partial + kMetricPartials * (size_t)lb);
 }
 
 __global__ void __launch_bounds__(kLearnBlock) kb_epe(BatchTable t, float *partial)

@@ -45,8 +45,12 @@ enum class LearnKernel { GatherInputs, Inject, Labels, Metrics, Epe };
 constexpr unsigned kLearnBlock = 256;
 inline unsigned learn_blocks(unsigned pixels) { return (pixels + kLearnBlock - 1) / kLearnBlock; }
 
+// partial sums per block of the Metrics reduction: energy, warped residual, pixels with
+// frame-1 depth, zero-motion residual
+constexpr int kMetricPartials = 4;
+
 // batched launch over a BatchTable of LearnArgs / MetricArgs / EpeArgs; the reductions write
-// per-block partial sums (Metrics: 3 floats, Epe: 2 floats) at partial + stride * blockIdx.x
+// per-block partial sums (Metrics: kMetricPartials floats, Epe: 2) at partial + stride * block
 hipError_t launch_learn(LearnKernel k, const BatchTable &t, unsigned total_blocks, float *partial,
                         hipStream_t s);
 

@@ -95,7 +95,7 @@ def hip_library():
         "pdf_tan_half_fovh": ([vp], c_float),
         "pdf_enable_kernel_timing": ([vp, c_int], c_int),
         "pdf_kernel_timing": ([vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
-                               ctypes.POINTER(ctypes.c_uint64)], c_int),
+                               ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)], c_int),
         "pdf_device_alloc": ([ctypes.POINTER(vp), ctypes.c_size_t], c_int),
         "pdf_device_free": ([vp], c_int),
         "pdf_memcpy_h2d": ([vp, vp, ctypes.c_size_t], c_int),
@@ -135,11 +135,13 @@ def hip_library():
         "pdf_batch_set_learned": ([vp, vp, up_, c_uint], c_int),
         "pdf_batch_metrics": ([vp, fp, fp], c_int),
         "pdf_batch_epe": ([vp, vp, fp], c_int),
+        "pdf_batch_quality": ([vp, up_, c_uint, fp, fp, fp], c_int),
+        "pdf_batch_fallback_bounds": ([vp, c_float, c_float, ctypes.POINTER(ctypes.c_ubyte), up_], c_int),
         "pdf_batch_resolve_plain": ([vp, up_, c_uint], c_int),
         "pdf_batch_fallback": ([vp, fp, fp, c_float, ctypes.POINTER(ctypes.c_ubyte), up_], c_int),
         "pdf_batch_learn_samples": ([vp, c_uint, c_uint, vp, vp, vp], c_int),
         "pdf_batch_kernel_timing": ([vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
-                                     ctypes.POINTER(ctypes.c_uint64)], c_int),
+                                     ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)], c_int),
     })
     for stage in ("assign_zeros", "upsample", "rij", "image_gradients", "warping",
                   "mu_and_step_sizes", "gradient", "dual_variables", "divergence",
@@ -368,9 +370,11 @@ class Solver:
         self._chk(self.lib.pdf_enable_kernel_timing(self.ctx, 1 if on else 0))
 
     def kernel_timing(self):
-        n, ms, pix = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_uint64()
-        self._chk(self.lib.pdf_kernel_timing(self.ctx, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(pix)))
-        return n.value, ms.value, pix.value
+        """(launches, total ms, pixel-iterations, pixels covered) of the timed iteration launches"""
+        n, ms, pix, pixl = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
+        self._chk(self.lib.pdf_kernel_timing(self.ctx, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(pix),
+                                             ctypes.byref(pixl)))
+        return n.value, ms.value, pix.value, pixl.value
 
 
 def solve_pair_noalloc(bufs, outs, rows=120, cam_mode=4, iters=None, stream=None, **overrides):
@@ -467,9 +471,11 @@ class Batch:
         self._chk(self.lib.pdf_batch_enable_kernel_timing(self.ctx, (2 if tb_only else 1) if on else 0))
 
     def kernel_timing(self):
-        n, ms, pix = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_uint64()
-        self._chk(self.lib.pdf_batch_kernel_timing(self.ctx, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(pix)))
-        return n.value, ms.value, pix.value
+        """(launches, total ms, pixel-iterations, pixels covered) of the timed iteration launches"""
+        n, ms, pix, pixl = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
+        self._chk(self.lib.pdf_batch_kernel_timing(self.ctx, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(pix),
+                                                   ctypes.byref(pixl)))
+        return n.value, ms.value, pix.value, pixl.value
 
     def inputs_device(self, pair):
         ptrs = [ctypes.POINTER(ctypes.c_float)() for _ in range(4)]
@@ -533,6 +539,27 @@ class Batch:
         r = np.empty(n, np.float32)
         self._chk(self.lib.pdf_batch_metrics(self.ctx, _fptr(e), _fptr(r)))
         return e, r
+
+    def quality(self, pairs=None):
+        """(energy, residual, zero-motion residual) of the listed pairs (all if None), in order
+        (pdf_batch_quality)"""
+        idx = list(range(len(self.rows))) if pairs is None else [int(i) for i in pairs]
+        n = len(idx)
+        e, r, r0 = (np.zeros(n, np.float32) for _ in range(3))
+        if n:
+            arr = None if pairs is None else (ctypes.c_uint * n)(*idx)
+            self._chk(self.lib.pdf_batch_quality(self.ctx, arr, 0 if pairs is None else n, _fptr(e), _fptr(r),
+                                                 _fptr(r0)))
+        return e, r, r0
+
+    def fallback_bounds(self, energy_per_pixel_max, residual_ratio_max):
+        """re-solves the pairs outside the bounds (pdf_batch_fallback_bounds); returns their flags"""
+        flags = np.zeros(len(self.rows), np.uint8)
+        cnt = ctypes.c_uint()
+        self._chk(self.lib.pdf_batch_fallback_bounds(self.ctx, energy_per_pixel_max, residual_ratio_max,
+                                                     flags.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)),
+                                                     ctypes.byref(cnt)))
+        return flags
 
     def epe(self, ref):
         out = np.empty(len(self.rows), np.float32)

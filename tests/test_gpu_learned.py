@@ -58,6 +58,53 @@ def test_metrics_match_numpy(pdflow):
         assert want_r < 20.0          # a converged solve explains most of the intensity change
 
 
+def test_quality_and_bounds_rule(pdflow):
+    """pdf_batch_quality: the energy and residual of pdf_batch_metrics plus the zero-motion
+    residual; pdf_batch_fallback_bounds: the calibrated rule re-solves exactly the pairs outside
+    its bounds, and with the committed calibration it rejects a corrupted network's pairs"""
+    import json
+    import os
+    b = _plain(pdflow)
+    e, r = b.metrics()
+    qe, qr, q0 = b.quality()
+    assert np.array_equal(e, qe) and np.array_equal(r, qr)
+    for i, rows in enumerate(ROWS):
+        I1, _, _ = b.pyramid(i, "colour_old", 0)
+        I2, _, _ = b.pyramid(i, "colour", 0)
+        Z1, _, _ = b.pyramid(i, "depth_old", 0)
+        zero = np.zeros(rows * rows * 4 // 3)
+        want = mt.residual(I1, I2, Z1, zero, zero, rows, rows * 4 // 3)
+        assert abs(q0[i] - want) <= 1e-4 * want, (i, q0[i], want)
+        assert qr[i] < q0[i]                 # the flow explains part of the intensity change
+    sub = b.quality([2, 0])
+    assert sub[0][0] == qe[2] and sub[2][1] == q0[0]
+    want = _flows(b, len(ROWS))
+    n = len(ROWS)
+    lb = pdflow.Batch(ROWS, iters=_cut(pdflow, ROWS))
+    _fill(pdflow, lb, ROWS)
+    lb.set_learned(_net(pdflow, n))
+    lb.solve(graph=True)
+    lb.synchronize()
+    learned = _flows(lb, n)
+    assert not lb.fallback_bounds(np.inf, np.inf).any()
+    for i in range(n):
+        assert np.array_equal(lb.flow(i)[0], learned[i][0])
+    assert lb.fallback_bounds(1e-6, 1e-6).all()
+    for i in range(n):
+        for c in range(3):
+            assert np.array_equal(lb.flow(i)[c], want[i][c]), (i, c)
+    rule = json.load(open(os.path.join(os.path.dirname(pdflow.__file__), "weights", "warmstart_qvga_rule.json")))
+    bad = pdflow.Batch(ROWS, iters=_cut(pdflow, ROWS))
+    _fill(pdflow, bad, ROWS)
+    bad.set_learned(_net(pdflow, n, seed=3, scale=3.0))
+    bad.solve(graph=True)
+    bad.synchronize()
+    flags = bad.fallback_bounds(rule["energy_per_pixel_max"], rule["residual_ratio_max"])
+    assert flags.all()
+    for i in range(n):
+        assert np.array_equal(bad.flow(i)[2], want[i][2])
+
+
 def test_epe_matches_numpy(pdflow):
     a = _plain(pdflow)
     assert np.all(a.epe(a) == 0)

@@ -80,3 +80,16 @@ def test_residual_and_epe_closed_forms():
     a = [np.zeros(rows * cols)] * 3
     b = [np.full(rows * cols, 0.3), np.full(rows * cols, 0.4), np.zeros(rows * cols)]
     assert abs(mt.epe(a, b, Z) - 0.5) < 1e-12
+
+
+def test_zero_motion_residual_is_the_mean_intensity_change():
+    """with zero flow the bilinear warp reads I2 at integer pixels: the residual is the mean of
+    |I2 - I1| over the pixels with frame-1 depth"""
+    rng = np.random.default_rng(5)
+    rows, cols = 9, 12
+    I1 = rng.uniform(0, 255, rows * cols).astype(np.float32)
+    I2 = rng.uniform(0, 255, rows * cols).astype(np.float32)
+    Z1 = np.where(rng.uniform(size=rows * cols) < 0.2, 0.0, 2.0).astype(np.float32)
+    zero = np.zeros(rows * cols)
+    want = np.abs(I2.astype(np.float64) - I1)[Z1 > 0].mean()
+    assert abs(mt.residual(I1, I2, Z1, zero, zero, rows, cols) - want) < 1e-9 * want
