@@ -132,7 +132,7 @@ __device__ __forceinline__ void pd_fused_body(const PDArgs &a, unsigned bx, unsi
         p.pwv = pin[5] + swy * ld * g2w;
         project(p.puu, p.puv);
         project(p.pvu, p.pvv);
-        project(p.pwu, p.pwv);
+        project_if_any(p.pwu, p.pwv);
         return p;
     };
     auto store_dual = [&](int h, const TVDual &p) {
@@ -277,22 +277,15 @@ __device__ __forceinline__ void derive_px(Px &q, float riu, float rjv, float li,
 }
 
 // dual step (.cu:813-887 with the gradient of .cu:982-1013): n* are (acc+prev) at (v, u+1),
-// m* at (v+1, u); read only when those pixels exist
-__device__ __forceinline__ void dual_px(Px &q, int rows, int cols, float nU, float nV, float nW,
-                                        float mU, float mV, float mW)
+// m* at (v+1, u).  In the last column (row) the reference's gradient is +0; the caller passes the
+// pixel's own values there (the same bits as u_c, v_c, w_c), whose difference is +0 and whose
+// product with the positive weight is +0 as well, so no border case is needed here (for finite
+// state values).
+__device__ __forceinline__ void dual_px(Px &q, float nU, float nV, float nW, float mU, float mV, float mW)
 {
     const float u_c = q.ua + q.up, v_c = q.va + q.vp, w_c = q.wa;
-    float g1u = 0.f, g1v = 0.f, g1w = 0.f, g2u = 0.f, g2v = 0.f, g2w = 0.f;
-    if (q.gu != cols - 1) {
-        g1u = q.ri * (nU - u_c);
-        g1v = q.ri * (nV - v_c);
-        g1w = q.ri * (nW - w_c);
-    }
-    if (q.gv != rows - 1) {
-        g2u = q.rj * (mU - u_c);
-        g2v = q.rj * (mV - v_c);
-        g2w = q.rj * (mW - w_c);
-    }
+    const float g1u = q.ri * (nU - u_c), g1v = q.ri * (nV - v_c), g1w = q.ri * (nW - w_c);
+    const float g2u = q.rj * (mU - u_c), g2v = q.rj * (mV - v_c), g2w = q.rj * (mW - w_c);
     q.p[0] = q.p[0] + q.sxl * g1u;
     q.p[1] = q.p[1] + q.syl * g2u;
     q.p[2] = q.p[2] + q.sxl * g1v;
@@ -301,24 +294,26 @@ __device__ __forceinline__ void dual_px(Px &q, int rows, int cols, float nU, flo
     q.p[5] = q.p[5] + q.swyl * g2w;
     project(q.p[0], q.p[1]);
     project(q.p[2], q.p[3]);
-    project(q.p[4], q.p[5]);
+    project_if_any(q.p[4], q.p[5]);
     q.pd = clamp_pd(q.pd + q.spdm * (-q.wa + q.ddt + q.ddu * q.ua + q.ddv * q.va));
 }
 
 // primal step (.cu:889-933 with the divergence of .cu:935-980): pu* are the new duals
 // puu, pvu, pwu at (v, u-1) with riu = ri there; pv* are puv, pvv, pwv at (v-1, u) with
-// rjv = rj there; read only when those pixels exist
+// rjv = rj there.  In the first column (row) the caller passes +0 for the weight and the duals
+// (an LDS slot that holds +0): the product is +0 and f - (+0) is f bit for bit, the
+// reference's first-column term.
 __device__ __forceinline__ void primal_px(Px &q, int rows, int cols, float riu, float puu,
                                           float pvu, float pwu, float rjv, float puv, float pvv,
                                           float pwv, float li, float ld)
 {
-    const bool uf = q.gu == 0, ul = q.gu == cols - 1, vf = q.gv == 0, vl = q.gv == rows - 1;
-    float divu = div_axis(q.ri, q.p[0], riu, puu, uf, ul);
-    float divv = div_axis(q.ri, q.p[2], riu, pvu, uf, ul);
-    float divw = div_axis(q.ri, q.p[4], riu, pwu, uf, ul);
-    divu += div_axis(q.rj, q.p[1], rjv, puv, vf, vl);
-    divv += div_axis(q.rj, q.p[3], rjv, pvv, vf, vl);
-    divw += div_axis(q.rj, q.p[5], rjv, pwv, vf, vl);
+    const bool ul = q.gu == cols - 1, vl = q.gv == rows - 1;
+    float divu = div_axis_last(q.ri, q.p[0], riu, puu, ul);
+    float divv = div_axis_last(q.ri, q.p[2], riu, pvu, ul);
+    float divw = div_axis_last(q.ri, q.p[4], riu, pwu, ul);
+    divu += div_axis_last(q.rj, q.p[1], rjv, puv, vl);
+    divv += div_axis_last(q.rj, q.p[3], rjv, pvv, vl);
+    divw += div_axis_last(q.rj, q.p[5], rjv, pwv, vl);
     const float du0 = q.du, dv0 = q.dv, dw0 = q.dw;
     const Primal p = primal_update(du0, dv0, dw0, q.tu, q.tv, q.tw, q.mu, q.pd, q.ddu, q.ddv, divu,
                                    divv, divw, q.dct, q.dcu, q.dcv, li, ld);
@@ -383,9 +378,10 @@ __device__ __forceinline__ int tb_pair_reach(int q)
 // neighbour exchange of one region (11 planes)
 template <int RV, int RU>
 struct TBLds {
+    static constexpr int kZero = RU * RV;      // a slot of ri, rj and the duals that holds +0
     float u[RU * RV], v[RU * RV], w[RU * RV];
-    float p[6][RU * RV];
-    float ri[RU * RV], rj[RU * RV];
+    float p[6][RU * RV + 1];
+    float ri[RU * RV + 1], rj[RU * RV + 1];
 };
 
 // where a thread sits in its region: a wave holds one column pair (slot) down all RV rows;
@@ -435,18 +431,21 @@ __device__ __forceinline__ void tb_iterate(const PDArgs &a, TBLds<RV, RU> &s, co
     // pixels outside the image run the same arithmetic on the values loaded for them; no pixel
     // inside reads their results (the image-border terms are selected away), and without a
     // branch the two pixels' instruction streams interleave
+    // neighbour slots: the pixel's own slot stands for (v, u+1) / (v+1, u) in the last column /
+    // row of the image (and at the region's edge, where only the discarded band reads it); the
+    // +0 slot stands for (v, u-1) / (v-1, u) in the first column / row (and at the region's edge)
     auto dual = [&](Px &q, int j) __attribute__((always_inline)) {
         const int lu = 2 * slot + j, idx = lu * RV + lv;
-        const int iu = lu + 1 < RU ? idx + RV : idx;
-        const int iv = lv + 1 < RV ? idx + 1 : idx;
-        dual_px(q, rows, cols, s.u[iu], s.v[iu], s.w[iu], s.u[iv], s.v[iv], s.w[iv]);
+        const int iu = lu + 1 < RU && q.gu != cols - 1 ? idx + RV : idx;
+        const int iv = lv + 1 < RV && q.gv != rows - 1 ? idx + 1 : idx;
+        dual_px(q, s.u[iu], s.v[iu], s.w[iu], s.u[iv], s.v[iv], s.w[iv]);
 #pragma unroll
         for (int k = 0; k < 6; k++) s.p[k][idx] = q.p[k];
     };
     auto primal = [&](Px &q, int j) __attribute__((always_inline)) {
         const int lu = 2 * slot + j, idx = lu * RV + lv;
-        const int ju = lu > 0 ? idx - RV : idx;
-        const int jv = lv > 0 ? idx - 1 : idx;
+        const int ju = lu > 0 && q.gu > 0 ? idx - RV : s.kZero;
+        const int jv = lv > 0 && q.gv > 0 ? idx - 1 : s.kZero;
         primal_px(q, rows, cols, s.ri[ju], s.p[0][ju], s.p[2][ju], s.p[4][ju], s.rj[jv], s.p[1][jv],
                   s.p[3][jv], s.p[5][jv], li, ld);
     };
@@ -485,7 +484,13 @@ __device__ __forceinline__ void tb_body(const PDArgs &a, unsigned bx, unsigned b
         s.rj[lu * RV + m.lv] = q.rj;
     };
     PD_EACH(load);
+    if (threadIdx.x < 8) {
+        // the +0 slot of ri, rj and the six dual planes (the dual steps never write it)
+        float *z = threadIdx.x < 6 ? s.p[threadIdx.x] : (threadIdx.x == 6 ? s.ri : s.rj);
+        z[s.kZero] = 0.f;
+    }
     __syncthreads();
+    // most regions lie inside the image: they iterate without the border cases
     tb_iterate<K, RV, RU>(a, s, m, x0, x1);
 }
 
@@ -498,8 +503,10 @@ constexpr int kLevelMaxPixels = 2 * BT;
 __device__ __forceinline__ void level_body(const PDArgs &a, int n)
 {
     __shared__ float sU[kLevelMaxPixels], sV[kLevelMaxPixels], sW[kLevelMaxPixels];
-    __shared__ float sP[6][kLevelMaxPixels];
-    __shared__ float sRi[kLevelMaxPixels], sRj[kLevelMaxPixels];
+    // one more slot of ri, rj and the duals, holding +0 (see primal_px)
+    constexpr int kZero = kLevelMaxPixels;
+    __shared__ float sP[6][kLevelMaxPixels + 1];
+    __shared__ float sRi[kLevelMaxPixels + 1], sRj[kLevelMaxPixels + 1];
     const int rows = (int)a.rows, cols = (int)a.cols, P = rows * cols;
     const float li = a.lambda_i, ld = a.lambda_d;
 
@@ -514,6 +521,10 @@ __device__ __forceinline__ void level_body(const PDArgs &a, int n)
         }
     };
     PD_BOTH(load);
+    if (threadIdx.x < 8) {
+        float *z = threadIdx.x < 6 ? sP[threadIdx.x] : (threadIdx.x == 6 ? sRi : sRj);
+        z[kZero] = 0.f;
+    }
     __syncthreads();
     // a pixel reads (v, u-1) only if u > 0 and (v-1, u) only if v > 0: indices stay in [0, P)
     auto derive = [&](Px &q, int j) __attribute__((always_inline)) {
@@ -534,15 +545,15 @@ __device__ __forceinline__ void level_body(const PDArgs &a, int n)
         if (!q.on) return;
         const int iu = q.gu != cols - 1 ? p + rows : p;
         const int iv = q.gv != rows - 1 ? p + 1 : p;
-        dual_px(q, rows, cols, sU[iu], sV[iu], sW[iu], sU[iv], sV[iv], sW[iv]);
+        dual_px(q, sU[iu], sV[iu], sW[iu], sU[iv], sV[iv], sW[iv]);
 #pragma unroll
         for (int k = 0; k < 6; k++) sP[k][p] = q.p[k];
     };
     auto primal = [&](Px &q, int j) __attribute__((always_inline)) {
         const int p = (int)threadIdx.x + j * BT;
         if (!q.on) return;
-        const int ju = q.gu > 0 ? p - rows : p;
-        const int jv = q.gv > 0 ? p - 1 : p;
+        const int ju = q.gu > 0 ? p - rows : kZero;
+        const int jv = q.gv > 0 ? p - 1 : kZero;
         primal_px(q, rows, cols, sRi[ju], sP[0][ju], sP[2][ju], sP[4][ju], sRj[jv], sP[1][jv],
                   sP[3][jv], sP[5][jv], li, ld);
     };

@@ -97,6 +97,15 @@ __device__ __forceinline__ void project(float &a, float &b)
     b = scale ? b * m : b;
 }
 
+// project() for a pair that seldom leaves the unit disc (the depth-flow duals pwu, pwv: about 1%
+// of the pixels): a wave in which no lane has a*a+b*b > 1 skips it, since project() changes
+// nothing then (m >= 1 for x <= 1, and a NaN x is raised to 1).  The test is wave-uniform.
+__device__ __forceinline__ void project_if_any(float &a, float &b)
+{
+    const float x = a * a + b * b;
+    if (__builtin_amdgcn_ballot_w64(x > 1.f)) project(a, b);
+}
+
 __device__ __forceinline__ float clamp_pd(float pd)
 {
     // .cu:837-846
@@ -110,6 +119,15 @@ __device__ __forceinline__ float div_axis(float r0, float p0, float rm, float pm
     const float f = r0 * p0, g = rm * pm;
     const float mid = f - g;
     return first ? f : (last ? -g : mid);
+}
+
+// div_axis without the first-element case: the caller passes rm = pm = +0 there, and f - (+0)
+// equals f bit for bit (for +0, -0 and every other f)
+__device__ __forceinline__ float div_axis_last(float r0, float p0, float rm, float pm, bool last)
+{
+    const float f = r0 * p0, g = rm * pm;
+    const float mid = f - g;
+    return last ? -g : mid;
 }
 
 struct Primal {
