@@ -91,6 +91,17 @@ __device__ __forceinline__ __attribute__((address_space(1))) T *gptr(T *p)
     return (__attribute__((address_space(1))) T *)p;
 }
 
+// Element i (< 2^30) of a plane in global memory, addressed as the plane's base plus a 32-bit
+// byte offset: with the base in scalar registers the access takes the scalar-base form of the
+// global instructions, so every plane of a pixel shares one offset register instead of a 64-bit
+// address per plane.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T &gat(T *p, unsigned i)
+{
+    auto b = (__attribute__((address_space(1))) char *)p;
+    return *(__attribute__((address_space(1))) T *)(b + (size_t)(i * (unsigned)sizeof(T)));
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2): block b runs
 // on XCD b % 8.  xcd_swizzle maps dispatch order to a tile order in which every XCD walks one
 // contiguous run of tiles, so neighbouring tiles (whose regions share halo rows and cache

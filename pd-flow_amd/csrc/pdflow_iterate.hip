@@ -212,16 +212,16 @@ __device__ __forceinline__ void load_state(Px &q, const PDArgs &a, int gv, int g
     q.on = on;
     const unsigned g = px_index(a, gv, gu, on);
     if constexpr (FRESH) {
-        q.dw = gptr(a.in[S_DW])[g];
+        q.dw = gat(a.in[S_DW], g);
         q.du = 0.f; q.dv = 0.f; q.ua = 0.f; q.va = 0.f; q.wa = q.dw;
     } else {
-        q.du = gptr(a.in[S_DU])[g]; q.dv = gptr(a.in[S_DV])[g]; q.dw = gptr(a.in[S_DW])[g];
-        q.ua = gptr(a.in[S_DU_ACC])[g]; q.va = gptr(a.in[S_DV_ACC])[g]; q.wa = gptr(a.in[S_DW_ACC])[g];
+        q.du = gat(a.in[S_DU], g); q.dv = gat(a.in[S_DV], g); q.dw = gat(a.in[S_DW], g);
+        q.ua = gat(a.in[S_DU_ACC], g); q.va = gat(a.in[S_DV_ACC], g); q.wa = gat(a.in[S_DW_ACC], g);
     }
-    q.pd = gptr(a.in[S_PD])[g];
+    q.pd = gat(a.in[S_PD], g);
 #pragma unroll
-    for (int k = 0; k < 6; k++) q.p[k] = gptr(a.in[S_PUU + k])[g];
-    q.ri = gptr(a.ri)[g]; q.rj = gptr(a.rj)[g];
+    for (int k = 0; k < 6; k++) q.p[k] = gat(a.in[S_PUU + k], g);
+    q.ri = gat(a.ri, g); q.rj = gat(a.rj, g);
 }
 
 // the other 9 per-level constant planes
@@ -245,18 +245,18 @@ __device__ __forceinline__ void load_px(Px &q, const PDArgs &a, int gv, int gu, 
     const unsigned g = px_index(a, gv, gu, on);
     float c[kConstPlanes];
 #pragma unroll
-    for (int k = 0; k < kConstPlanes; k++) c[k] = gptr(const_plane(a, k))[g];
+    for (int k = 0; k < kConstPlanes; k++) c[k] = gat(const_plane(a, k), g);
     set_consts(q, c);
 }
 
 __device__ __forceinline__ void store_px(const Px &q, const PDArgs &a)
 {
     const unsigned g = (unsigned)q.gv + (unsigned)q.gu * a.rows;
-    gptr(a.out[S_DU])[g] = q.du; gptr(a.out[S_DV])[g] = q.dv; gptr(a.out[S_DW])[g] = q.dw;
-    gptr(a.out[S_DU_ACC])[g] = q.ua; gptr(a.out[S_DV_ACC])[g] = q.va; gptr(a.out[S_DW_ACC])[g] = q.wa;
-    gptr(a.out[S_PD])[g] = q.pd;
+    gat(a.out[S_DU], g) = q.du; gat(a.out[S_DV], g) = q.dv; gat(a.out[S_DW], g) = q.dw;
+    gat(a.out[S_DU_ACC], g) = q.ua; gat(a.out[S_DV_ACC], g) = q.va; gat(a.out[S_DW_ACC], g) = q.wa;
+    gat(a.out[S_PD], g) = q.pd;
 #pragma unroll
-    for (int k = 0; k < 6; k++) gptr(a.out[S_PUU + k])[g] = q.p[k];
+    for (int k = 0; k < 6; k++) gat(a.out[S_PUU + k], g) = q.p[k];
 }
 
 // riu = ri at (v, u-1), rjv = rj at (v-1, u); only read when those pixels exist
