@@ -103,8 +103,10 @@ int pdf_pd_iterations(pdf_ctx *ctx, unsigned n);
 
 /* Iteration-kernel selection (no reference counterpart; the reference launches four kernels
  * per iteration): levels of at least min_pixels pixels run k iterations per launch in the
- * temporally blocked kernel (k in {2,3,4,5,6,8}); k <= 1 disables it.  Results are identical
- * for every setting.  Default: k = 6 from 60x80 up (batches: k = 4, pdf_batch_set_iteration_kernel). */
+ * temporally blocked kernel (k in {2,3,4,5,6,8,10,12}); k <= 1 disables it.  Results are
+ * identical for every setting.  Default: from 60x80 up, 12 iterations per launch on levels of at
+ * most 60x80 pixels, 10 up to 120x160 and 6 above (batches: k = 4 on every level,
+ * pdf_batch_set_iteration_kernel); calling this function replaces the per-level choice by k. */
 int pdf_set_iteration_kernel(pdf_ctx *ctx, int k, unsigned min_pixels);
 /* Region shape of the temporally blocked kernel (no reference counterpart): 0 = 64 rows x 32
  * columns per 1024-thread workgroup, 1 = 32 x 32 per 512-thread workgroup (default: two

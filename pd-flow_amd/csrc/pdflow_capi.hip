@@ -97,6 +97,9 @@ struct pdf_ctx {
     // iteration kernel selection: levels of at least tb_min_pixels pixels run tb_k
     // iterations per launch (temporal blocking); 0 disables it
     int tb_k = 6;
+    // the default per-level choice (tb_k_for): more iterations per launch on small levels, whose
+    // launches take about the same time whatever their work; off once the caller picks k
+    bool tb_auto = true;
     int tb_shape = pdf::kTBShape32x32;
     unsigned tb_min_pixels = 60 * 80;
     // levels of at most 2048 pixels run all their iterations in one workgroup
@@ -523,6 +526,18 @@ static pdf::PDArgs pd_args(const pdf_ctx *c)
     return a;
 }
 
+// Iterations per launch of a level of P pixels for one pair: a level of up to 60x80 or 120x160
+// pixels fills only a few dozen workgroups, and its launches take about the same time for any k,
+// so it runs 12 or 10 iterations per launch; larger levels run k (measured on MI355X, reference
+// schedule at 480x640: 60x80 k = 12 0.087 ms against 0.113 for k = 6, 120x160 k = 10 0.111
+// against 0.137; 240x320 and 480x640 fastest at k = 6)
+static int tb_k_for(size_t P, int k)
+{
+    if (P <= 60 * 80) return 12;
+    if (P <= 120 * 160) return 10;
+    return k;
+}
+
 // one launch doing `k` iterations (k == 1: single-iteration kernel)
 static int pd_launch(pdf_ctx *c, int k)
 {
@@ -571,8 +586,10 @@ int pdf_pd_iterations(pdf_ctx *c, unsigned n)
     if (rc) return rc;
     if (n == 0) return PDF_OK;
     if (c->level_kernel && pdf::pd_level_supported(c->rows_i * c->cols_i)) return pd_launch_level(c, n);
-    const bool tb = c->tb_k > 1 && (size_t)c->rows_i * c->cols_i >= c->tb_min_pixels;
-    return pdf::for_each_tb_chunk(n, tb ? c->tb_k : 1, [c](int k) { return pd_launch(c, k); });
+    const size_t P = (size_t)c->rows_i * c->cols_i;
+    const bool tb = c->tb_k > 1 && P >= c->tb_min_pixels;
+    const int K = !tb ? 1 : (c->tb_auto ? tb_k_for(P, c->tb_k) : c->tb_k);
+    return pdf::for_each_tb_chunk(n, K, [c](int k) { return pd_launch(c, k); });
 }
 
 int pdf_set_iteration_kernel(pdf_ctx *c, int k, unsigned min_pixels)
@@ -580,6 +597,7 @@ int pdf_set_iteration_kernel(pdf_ctx *c, int k, unsigned min_pixels)
     if (!c) return PDF_ERR_ARG;
     if (k > 1 && !pdf::pd_tb_supported(k)) return fail(c, PDF_ERR_ARG, "unsupported iterations per launch");
     c->tb_k = k > 1 ? k : 0;
+    c->tb_auto = false;
     c->tb_min_pixels = min_pixels;
     for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
     c->graphs.clear();

@@ -140,7 +140,7 @@ constexpr int kFilterNoCarry = 1;
 constexpr int kInitCoarse = 1, kInitSkipConst = 2;
 
 // Iterations per launch supported by the temporally blocked kernel.
-constexpr int kTBMaxK = 8;
+constexpr int kTBMaxK = 12;
 bool pd_tb_supported(int k);
 
 // How n iterations of a level are split into launches of the temporally blocked kernel with k

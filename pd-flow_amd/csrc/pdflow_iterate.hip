@@ -629,7 +629,7 @@ hipError_t launch_pd_level(const PDArgs &a, unsigned n, hipStream_t s)
 
 bool pd_level_supported(unsigned pixels) { return pixels <= (unsigned)kLevelMaxPixels; }
 
-bool pd_tb_supported(int k) { return k == 2 || k == 3 || k == 4 || k == 5 || k == 6 || k == 8; }
+bool pd_tb_supported(int k) { return k == 2 || k == 3 || k == 4 || k == 5 || k == 6 || k == 8 || k == 10 || k == 12; }
 bool pd_tb_shape_supported(int shape) { return shape == kTBShape64x32 || shape == kTBShape32x32; }
 
 template <int RV, int RU>
@@ -658,6 +658,8 @@ static hipError_t launch_tb_shape(const PDArgs &a, int k, hipStream_t s)
     case 5: return launch_tb<5, RV, RU>(a, s);
     case 6: return launch_tb<6, RV, RU>(a, s);
     case 8: return launch_tb<8, RV, RU>(a, s);
+    case 10: return launch_tb<10, RV, RU>(a, s);
+    case 12: return launch_tb<12, RV, RU>(a, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -697,6 +699,8 @@ static hipError_t launch_tb_batch(int k, const BatchTable &t, const dim3 &grid, 
     case 5: hipLaunchKernelGGL((kb_pd_tb<5, RV, RU, FRESH>), grid, block, 0, s, t); break;
     case 6: hipLaunchKernelGGL((kb_pd_tb<6, RV, RU, FRESH>), grid, block, 0, s, t); break;
     case 8: hipLaunchKernelGGL((kb_pd_tb<8, RV, RU, FRESH>), grid, block, 0, s, t); break;
+    case 10: hipLaunchKernelGGL((kb_pd_tb<10, RV, RU, FRESH>), grid, block, 0, s, t); break;
+    case 12: hipLaunchKernelGGL((kb_pd_tb<12, RV, RU, FRESH>), grid, block, 0, s, t); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
