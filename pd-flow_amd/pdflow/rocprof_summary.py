@@ -1,0 +1,111 @@
+"""Summaries of rocprofv3 output for profiles/ (measurement tooling; runs on the CSV files only).
+
+``traffic``: HBM bytes per launch of the batched iteration kernel from two ``rocprofv3 --pmc``
+runs of the same bench command (FETCH_SIZE, then WRITE_SIZE), each scaled by what the counter read
+for the plain copy kernel of known size that ``bench.py --copy-calibration BYTES`` runs first
+(every byte read once and written once).  The launches summarised are the last ``--launches``
+dispatches of kb_pd_tb, the eager solves whose HIP-event timing gives bench.py's roofline.  The
+last line, ``traffic: bytes_per_launch=... workload=... iterate_source_sha16=...``, is the one
+bench.py reads back.
+
+``trace``: the average duration of those launches in a ``--kernel-trace`` CSV of the same command,
+to set beside bench.py's HIP-event average.
+
+    python -m pdflow.rocprof_summary traffic FETCH.csv WRITE.csv --copy-bytes 1073741824 \\
+        --launches 117 --workload KEY
+    python -m pdflow.rocprof_summary trace TRACE.csv --launches 117
+"""
+import argparse
+import collections
+import csv
+import hashlib
+import os
+
+ITERATE_SRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "csrc",
+                           "pdflow_iterate.hip")
+
+
+def _rows(path):
+    with open(path) as f:
+        return list(csv.DictReader(f))
+
+
+def _sorted_dispatches(rows):
+    return sorted(rows, key=lambda r: int(r["Dispatch_Id"]) if "Dispatch_Id" in r else int(r["Start_Timestamp"]))
+
+
+def _factor(rows, copy_bytes):
+    """bytes per counter KiB: the copy kernel moves copy_bytes each way"""
+    vals = [float(r["Counter_Value"]) for r in rows if "k_stream_copy" in r["Kernel_Name"]]
+    if not vals:
+        raise SystemExit("no calibration copy in the counter file")
+    return copy_bytes / (sum(vals) / len(vals) * 1024.0), vals
+
+
+def traffic(a):
+    fetch, write = _sorted_dispatches(_rows(a.fetch)), _sorted_dispatches(_rows(a.write))
+    ff, fv = _factor(fetch, a.copy_bytes)
+    wf, wv = _factor(write, a.copy_bytes)
+    tb_f = [r for r in fetch if "kb_pd_tb" in r["Kernel_Name"]][-a.launches:]
+    tb_w = [r for r in write if "kb_pd_tb" in r["Kernel_Name"]][-a.launches:]
+    if len(tb_f) != a.launches or len(tb_w) != a.launches:
+        raise SystemExit("fewer kb_pd_tb dispatches than --launches")
+    kinds = collections.OrderedDict()
+    for rf, rw in zip(tb_f, tb_w):
+        if rf["Kernel_Name"] != rw["Kernel_Name"] or rf["Grid_Size"] != rw["Grid_Size"]:
+            raise SystemExit("the two runs dispatched different launches")
+        k = (rf["Kernel_Name"].split("kb_pd_tb")[1].split("(")[0], int(rf["Grid_Size"]))
+        kinds.setdefault(k, []).append((float(rf["Counter_Value"]) * 1024 * ff, float(rw["Counter_Value"]) * 1024 * wf))
+    print(f"Calibration: k_stream_copy of {a.copy_bytes} bytes, {len(fv)} dispatches: FETCH_SIZE "
+          f"{sum(fv) / len(fv):,.0f} KiB -> factor {ff:.4f}; WRITE_SIZE {sum(wv) / len(wv):,.0f} KiB -> "
+          f"factor {wf:.4f} (bytes = counter KiB x 1024 x factor)")
+    print()
+    print("| instance | grid (threads) | launches | read GB | write GB | HBM GB per launch |")
+    print("|---|---|---|---|---|---|")
+    tot = 0.0
+    for (name, grid), v in kinds.items():
+        r = sum(x for x, _ in v) / len(v)
+        w = sum(y for _, y in v) / len(v)
+        tot += (r + w) * len(v)
+        print(f"| kb_pd_tb{name} | {grid} | {len(v)} | {r / 1e9:.3f} | {w / 1e9:.3f} | {(r + w) / 1e9:.3f} |")
+    per = tot / a.launches
+    print()
+    print(f"Average over the {a.launches} launches: {per / 1e9:.4f} GB of HBM traffic per launch.")
+    with open(ITERATE_SRC, "rb") as f:
+        sha = hashlib.sha256(f.read()).hexdigest()[:16]
+    print()
+    print(f"traffic: bytes_per_launch={per:.6e} workload={a.workload} iterate_source_sha16={sha}")
+
+
+def trace(a):
+    rows = sorted(_rows(a.trace), key=lambda r: int(r["Start_Timestamp"]))
+    tb = [r for r in rows if "kb_pd_tb" in r["Kernel_Name"]][-a.launches:]
+    dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in tb]
+    ov = sum(1 for x, y in zip(tb, tb[1:]) if int(y["Start_Timestamp"]) < int(x["End_Timestamp"]))
+    print(f"last {len(tb)} kb_pd_tb dispatches: average {sum(dur) / len(dur):.1f} us, {ov} overlapping pairs")
+    by = collections.OrderedDict()
+    for r, d in zip(tb, dur):
+        k = (r["Kernel_Name"].split("kb_pd_tb")[1].split("(")[0], int(r.get("Grid_Size_X") or r.get("Grid_Size")))
+        by.setdefault(k, []).append(d)
+    for (name, grid), v in by.items():
+        print(f"  kb_pd_tb{name} grid {grid}: {len(v)} launches, average {sum(v) / len(v):.1f} us")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch")
+    t.add_argument("write")
+    t.add_argument("--copy-bytes", type=int, required=True)
+    t.add_argument("--launches", type=int, required=True)
+    t.add_argument("--workload", required=True)
+    r = sub.add_parser("trace")
+    r.add_argument("trace")
+    r.add_argument("--launches", type=int, required=True)
+    a = ap.parse_args(argv)
+    (traffic if a.cmd == "traffic" else trace)(a)
+
+
+if __name__ == "__main__":
+    main()
