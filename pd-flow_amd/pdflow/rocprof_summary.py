@@ -11,9 +11,14 @@ bench.py reads back.
 ``trace``: the average duration of those launches in a ``--kernel-trace`` CSV of the same command,
 to set beside bench.py's HIP-event average.
 
+``kernels``: every dispatch of a ``--kernel-trace`` CSV grouped by kernel and grid, with totals
+divided by ``--per`` (the number of solves or pairs traced), and the average of the last
+``--iteration-launches`` dispatches of the single-pair iteration kernels.
+
     python -m pdflow.rocprof_summary traffic FETCH.csv WRITE.csv --copy-bytes 1073741824 \\
         --launches 117 --workload KEY
     python -m pdflow.rocprof_summary trace TRACE.csv --launches 117
+    python -m pdflow.rocprof_summary kernels TRACE.csv --per 43 --iteration-launches 940
 """
 import argparse
 import collections
@@ -91,9 +96,58 @@ def trace(a):
         print(f"  kb_pd_tb{name} grid {grid}: {len(v)} launches, average {sum(v) / len(v):.1f} us")
 
 
+ITERATION_KERNELS = ("k_pd_tb", "k_pd_level", "k_pd_fused")
+
+
+def _short(name):
+    """kernel name without return type, namespaces and parameter list"""
+    name = name[5:] if name.startswith("void ") else name
+    if name.endswith(")"):                 # drop the parameter list: the '(' matching the last ')'
+        depth = 0
+        for i in range(len(name) - 1, -1, -1):
+            depth += {")": 1, "(": -1}.get(name[i], 0)
+            if depth == 0:
+                name = name[:i]
+                break
+    for ns in ("pdf::cnn::(anonymous namespace)::", "pdf::(anonymous namespace)::", "pdf::"):
+        name = name.replace(ns, "")
+    return name
+
+
+def kernels(a):
+    rows = sorted(_rows(a.trace), key=lambda r: int(r["Start_Timestamp"]))
+    groups = collections.OrderedDict()
+    for r in rows:
+        k = (_short(r["Kernel_Name"]), int(r.get("Grid_Size_X") or r.get("Grid_Size")))
+        groups.setdefault(k, []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    print(f"{len(rows)} dispatches; per unit = total / {a.per}")
+    print()
+    print("| kernel | grid x (threads) | dispatches | average us | per unit ms |")
+    print("|---|---|---|---|---|")
+    tot = 0.0
+    for (name, grid), v in sorted(groups.items(), key=lambda kv: -sum(kv[1])):
+        tot += sum(v)
+        if sum(v) / a.per / 1e3 < a.min_ms:
+            continue
+        print(f"| {name} | {grid} | {len(v)} | {sum(v) / len(v):.2f} | {sum(v) / a.per / 1e3:.4f} |")
+    print()
+    print(f"All dispatches: {tot / a.per / 1e3:.4f} ms per unit (rows under {a.min_ms} ms per unit omitted above).")
+    if a.iteration_launches:
+        it = [r for r in rows if _short(r["Kernel_Name"]).startswith(ITERATION_KERNELS)][-a.iteration_launches:]
+        dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in it]
+        print()
+        print(f"last {len(it)} iteration-kernel dispatches ({', '.join(ITERATION_KERNELS)}): average "
+              f"{sum(dur) / len(dur):.2f} us")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
+    g = sub.add_parser("kernels")
+    g.add_argument("trace")
+    g.add_argument("--per", type=float, default=1.0, help="divide totals by this count (solves, pairs)")
+    g.add_argument("--min-ms", type=float, default=0.0)
+    g.add_argument("--iteration-launches", type=int, default=0)
     t = sub.add_parser("traffic")
     t.add_argument("fetch")
     t.add_argument("write")
@@ -104,7 +158,7 @@ def main(argv=None):
     r.add_argument("trace")
     r.add_argument("--launches", type=int, required=True)
     a = ap.parse_args(argv)
-    (traffic if a.cmd == "traffic" else trace)(a)
+    {"traffic": traffic, "trace": trace, "kernels": kernels}[a.cmd](a)
 
 
 if __name__ == "__main__":
