@@ -167,6 +167,8 @@ def main():
                          "iterations from where they stand (pdf_batch_continue), stage by stage for "
                          "a list, each stage re-checking the rule; the plain path then re-solves "
                          "those still outside (0: no continuation)")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="learned: skip the serving rate (inputs in and flows out over PCIe every step)")
     ap.add_argument("--pipeline", type=int, choices=(0, 1), default=1,
                     help="learned: 1 = two batches in flight on their own streams, so that one batch's "
                          "continuation and fallback overlap the next batch's solve; 0 = one batch, "
@@ -511,6 +513,52 @@ def run_learned(args):
     log(f"bench: {args.steps} learned steps in {el:.3f} s -> {pairs_s:.1f} pairs/s "
         f"(fallback {int(flags.sum())}/{n})")
 
+    # serving: each step also copies its pairs' inputs in from page-locked host memory before the
+    # solve and their flows out after the rule, on the batch's stream, so that one batch's copies
+    # overlap the other batch's kernels (the PCIe-inclusive rate; `value` keeps the inputs resident)
+    serving = None
+    if not args.no_serving:
+        from pdflow import PinnedBuffer
+        src = PinnedBuffer(n * 4 * P)
+        src.array[:] = np.stack([np.ascontiguousarray(p, np.float32).reshape(n, -1) for p in planes], axis=1).ravel()
+        dsts = [PinnedBuffer(n * 3 * P) for _ in batches]
+
+        def run_serving(k):
+            """run_steps with the copies: a batch's inputs go in before its solve, its flows come
+            out after its finish"""
+            nb = len(batches)
+
+            def start(b):
+                b.upload_inputs_async(src.array)
+                b.solve(graph=not args.no_graph)
+
+            start(batches[0])
+            for s in range(k):
+                cur = batches[s % nb]
+                if nb > 1 and s + 1 < k:
+                    start(batches[(s + 1) % nb])
+                finish(cur)
+                cur.download_flows_async(dsts[s % nb].array)
+                if nb == 1 and s + 1 < k:
+                    start(cur)
+            for b in batches:
+                b.synchronize()
+
+        run_serving(max(1, args.warmup))
+        t0 = time.perf_counter()
+        run_serving(args.steps)
+        serving = n * args.steps / (time.perf_counter() - t0)
+        # the flows that came back are the batch's own
+        last = batches[(args.steps - 1) % len(batches)]
+        got = dsts[(args.steps - 1) % len(batches)].array
+        for i in (0, n - 1):
+            for k, w in enumerate(last.flow(i)):
+                if not np.array_equal(got[(3 * i + k) * P:(3 * i + k + 1) * P], w):
+                    raise SystemExit("bench.py: a flow copied out asynchronously differs from the batch's")
+        log(f"bench: serving (inputs in and flows out over PCIe every step) {serving:.1f} pairs/s")
+        for buf in [src] + dsts:
+            buf.free()
+
     # the rule as the configuration words it: every step also solves every pair with the plain
     # path, whose energy and residual are the references; pairs more than 5% above them get the
     # same continuation stages, then the plain re-solve
@@ -626,6 +674,8 @@ def run_learned(args):
                      "plain_batched_pairs_s": round(plain_batched, 3),
                      "learned_without_rule_pairs_s": round(learned_only, 3),
                      "rule_as_worded_pairs_s": round(rule_as_worded, 3),
+                     "serving_pairs_s": None if serving is None else round(serving, 3),
+                     "serving_pcie_bytes_per_pair": {"to_device": 16 * P, "to_host": 12 * P},
                      "speedup_vs_plain_single_pair": round(pairs_s / plain_single, 3),
                      "speedup_vs_plain_batched": round(pairs_s / plain_batched, 3)},
         "rule": {"accepted_fraction": round(float(self_ok.mean()), 4),

@@ -266,6 +266,20 @@ int pdf_batch_synchronize(pdf_batch *b);
 int pdf_batch_flow_device(pdf_batch *b, unsigned pair, const float **dx, const float **dy,
                           const float **dz, unsigned *rows, unsigned *cols);
 int pdf_batch_copy_flow(pdf_batch *b, unsigned pair, float *dx, float *dy, float *dz);
+/* No counterpart in the reference (its copyNewFrames, pdflow_cudalib.cu:195-222, copies one frame
+ * and blocks): queues the copy of every pair's four input planes from host memory on the batch's
+ * stream and returns at once.  host holds, pair after pair in batch order, I1, Z1, I2, Z2 of
+ * rows x cols floats each (column-major); host_floats must equal that total.  The host memory must
+ * stay unchanged until the stream has passed the copy (pdf_batch_synchronize, or any call that
+ * waits on the stream); from page-locked memory (pdf_host_alloc) the copy overlaps kernels on
+ * other streams.  Queue it after the previous solve's results are used and before the next solve,
+ * as for pdf_batch_inputs_device. */
+int pdf_batch_upload_inputs_async(pdf_batch *b, const float *host, size_t host_floats);
+/* No counterpart in the reference (its copyMotionField, pdflow_cudalib.cu:239-244, blocks): queues
+ * the copy of every pair's finest-level flow, dx, dy, dz of rows x cols floats each, pair after
+ * pair, into host memory on the batch's stream and returns at once; the values are there once the
+ * stream has passed the copy. */
+int pdf_batch_download_flows_async(pdf_batch *b, float *host, size_t host_floats);
 /* number of kernel launches one batched solve makes */
 int pdf_batch_launch_count(const pdf_batch *b, unsigned *launches);
 /* HIP-event timing of the iteration launches, as pdf_enable_kernel_timing (solves run
@@ -344,6 +358,9 @@ int pdf_device_alloc(void **ptr, size_t bytes);
 int pdf_device_free(void *ptr);
 int pdf_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int pdf_memcpy_d2h(void *dst, const void *src, size_t bytes);
+/* page-locked host memory (hipHostMalloc), for copies that overlap kernels */
+int pdf_host_alloc(void **ptr, size_t bytes);
+int pdf_host_free(void *ptr);
 /* blocking device-to-device copy by a plain streaming kernel (bytes a multiple of 16, 16-byte
  * aligned pointers): every byte read once and written once, the calibration point for HBM byte
  * counters */

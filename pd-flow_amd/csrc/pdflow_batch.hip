@@ -1033,6 +1033,49 @@ int pdf_batch_copy_flow(pdf_batch *b, unsigned pair, float *dx, float *dy, float
     return PDF_OK;
 }
 
+// copies between host memory laid out pair after pair and `n` planes of each pair; planes that
+// lie back to back in the arena (the usual case: they are carved in this order, with no padding
+// when the plane size is a multiple of the alignment) go in one copy
+static int batch_copy_planes(pdf_batch *b, float *host, size_t host_floats, int n, bool to_device,
+                             float *(*plane)(Pair &, int), size_t (*plane_floats)(const Pair &))
+{
+    if (!b) return PDF_ERR_ARG;
+    if (!host) return bfail(b, PDF_ERR_ARG, "null host pointer");
+    size_t need = 0;
+    for (const Pair &p : b->pairs) need += (size_t)n * plane_floats(p);
+    if (host_floats != need) return bfail(b, PDF_ERR_ARG, "host buffer size does not match the batch's planes");
+    const hipMemcpyKind kind = to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    float *h = host;
+    for (Pair &p : b->pairs) {
+        const size_t P0 = plane_floats(p);
+        bool adjacent = true;
+        for (int k = 1; k < n; k++) adjacent = adjacent && plane(p, k) == plane(p, 0) + k * P0;
+        const int copies = adjacent ? 1 : n;
+        const size_t floats = adjacent ? n * P0 : P0;
+        for (int k = 0; k < copies; k++) {
+            float *dev = plane(p, k), *hk = h + k * P0;
+            BHIP(b, hipMemcpyAsync(to_device ? (void *)dev : (void *)hk, to_device ? (const void *)hk : (const void *)dev,
+                                   floats * sizeof(float), kind, b->stream));
+        }
+        h += n * P0;
+    }
+    return PDF_OK;
+}
+
+int pdf_batch_upload_inputs_async(pdf_batch *b, const float *host, size_t host_floats)
+{
+    return batch_copy_planes(b, const_cast<float *>(host), host_floats, 4, true,
+                             [](Pair &p, int k) { return p.input[k]; },
+                             [](const Pair &p) { return (size_t)p.width * p.height; });
+}
+
+int pdf_batch_download_flows_async(pdf_batch *b, float *host, size_t host_floats)
+{
+    return batch_copy_planes(b, host, host_floats, 3, false,
+                             [](Pair &p, int k) { return p.f[kFlowPlanes[k]]; },
+                             [](const Pair &p) { return (size_t)p.rows * p.cols; });
+}
+
 int pdf_batch_launch_count(const pdf_batch *b, unsigned *launches)
 {
     if (!b || !launches) return PDF_ERR_ARG;

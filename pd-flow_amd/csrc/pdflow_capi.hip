@@ -920,6 +920,15 @@ int pdf_memcpy_d2h(void *dst, const void *src, size_t bytes)
     return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? PDF_OK : PDF_ERR_HIP;
 }
 
+int pdf_host_alloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return PDF_ERR_ARG;
+    *ptr = nullptr;
+    return hipHostMalloc(ptr, bytes, hipHostMallocDefault) == hipSuccess ? PDF_OK : PDF_ERR_NOMEM;
+}
+
+int pdf_host_free(void *ptr) { return hipHostFree(ptr) == hipSuccess ? PDF_OK : PDF_ERR_HIP; }
+
 int pdf_device_copy(void *dst, const void *src, size_t bytes)
 {
     if (!dst || !src || bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16) return PDF_ERR_ARG;

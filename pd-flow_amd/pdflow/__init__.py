@@ -11,6 +11,7 @@ from .lib import (  # noqa: F401
     Batch,
     PYR_KINDS,
     PDFlowError,
+    PinnedBuffer,
     Solver,
     default_params,
     device_count,

@@ -100,6 +100,8 @@ def hip_library():
         "pdf_device_free": ([vp], c_int),
         "pdf_memcpy_h2d": ([vp, vp, ctypes.c_size_t], c_int),
         "pdf_memcpy_d2h": ([vp, vp, ctypes.c_size_t], c_int),
+        "pdf_host_alloc": ([ctypes.POINTER(vp), ctypes.c_size_t], c_int),
+        "pdf_host_free": ([vp], c_int),
         "pdf_device_count": ([ctypes.POINTER(c_int)], c_int),
     }
     up_ = ctypes.POINTER(ctypes.c_uint)
@@ -116,6 +118,8 @@ def hip_library():
         "pdf_batch_synchronize": ([vp], c_int),
         "pdf_batch_flow_device": ([vp, c_uint, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), up_, up_], c_int),
         "pdf_batch_copy_flow": ([vp, c_uint, fp, fp, fp], c_int),
+        "pdf_batch_upload_inputs_async": ([vp, fp, ctypes.c_size_t], c_int),
+        "pdf_batch_download_flows_async": ([vp, fp, ctypes.c_size_t], c_int),
         "pdf_batch_launch_count": ([vp, up_], c_int),
         "pdf_batch_set_start_level": ([vp, c_uint, c_uint, fpp], c_int),
         "pdf_batch_set_schedule": ([vp, c_uint, up_], c_int),
@@ -215,6 +219,31 @@ class DeviceBuffer:
     def free(self):
         if self.ptr:
             self.lib.pdf_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class PinnedBuffer:
+    """Page-locked host memory (pdf_host_alloc) seen as a float32 NumPy array of `floats`
+    elements; copies from and to it can overlap kernels."""
+
+    def __init__(self, floats):
+        self.lib = hip_library()
+        self.ptr = ctypes.c_void_p()
+        rc = self.lib.pdf_host_alloc(ctypes.byref(self.ptr), floats * 4)
+        if rc:
+            raise PDFlowError(rc, f"page-locked allocation of {floats * 4} bytes")
+        self.array = np.ctypeslib.as_array(ctypes.cast(self.ptr, ctypes.POINTER(ctypes.c_float)), shape=(floats,))
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            self.lib.pdf_host_free(self.ptr)
             self.ptr = ctypes.c_void_p()
 
     def __del__(self):
@@ -459,6 +488,17 @@ class Batch:
 
     def synchronize(self):
         self._chk(self.lib.pdf_batch_synchronize(self.ctx))
+
+    def upload_inputs_async(self, host):
+        """queues the copy of every pair's I1, Z1, I2, Z2 (pair after pair) from the float32 array
+        `host` (best a PinnedBuffer's) on the batch's stream; `host` must stay unchanged until the
+        stream has passed the copy"""
+        self._chk(self.lib.pdf_batch_upload_inputs_async(self.ctx, _fptr(host), host.size))
+
+    def download_flows_async(self, host):
+        """queues the copy of every pair's dx, dy, dz (pair after pair) into the float32 array
+        `host` on the batch's stream"""
+        self._chk(self.lib.pdf_batch_download_flows_async(self.ctx, _fptr(host), host.size))
 
     def flow(self, pair):
         r = self.rows[pair]

@@ -4,7 +4,7 @@ import sys
 import numpy as np
 import pytest
 
-from pdflow import Batch, Solver, synth_pair_float
+from pdflow import Batch, PDFlowError, PinnedBuffer, Solver, synth_pair_float
 
 pytestmark = pytest.mark.gpu
 
@@ -222,3 +222,43 @@ def test_full_size_batch_matches_single_pairs(gpu_available):
         want = g.solve_pair(*(p[i] for p in planes))
         for a, w in zip(b.flow(i), want):
             np.testing.assert_array_equal(a, w, err_msg=f"pair {i}")
+
+
+def test_async_inputs_and_flows_match_the_blocking_copies(gpu_available):
+    """pdf_batch_upload_inputs_async / pdf_batch_download_flows_async from page-locked memory: new
+    inputs queued after a solve take effect in the next solve, and the flows copied out equal the
+    blocking per-pair copies of a batch given the same inputs by pdf_batch_set_inputs."""
+    rows = [480, 240, 120, 60, 240]
+    say(f"async copies, batch of {rows}")
+    first = [single(r, 700 + i)[0] for i, r in enumerate(rows)]
+    second = [single(r, 800 + i)[0] for i, r in enumerate(rows)]
+    b, ref = Batch(rows), Batch(rows)
+    for i in range(len(rows)):
+        b.set_inputs(i, *first[i])
+        ref.set_inputs(i, *second[i])
+    b.solve(graph=True)
+    ref.solve(graph=True)
+    ref.synchronize()
+    n_in = sum(4 * r * (r * 4 // 3) for r in rows)
+    n_out = sum(3 * r * (r * 4 // 3) for r in rows)
+    src, dst = PinnedBuffer(n_in), PinnedBuffer(n_out)
+    src.array[:] = np.concatenate([np.ascontiguousarray(x, np.float32).ravel() for pair in second for x in pair])
+    dst.array[:] = np.nan
+    with pytest.raises(PDFlowError):
+        b.upload_inputs_async(src.array[:-1])
+    with pytest.raises(PDFlowError):
+        b.download_flows_async(dst.array[1:])
+    for rep in range(2):
+        b.upload_inputs_async(src.array)
+        b.solve(graph=True)
+        b.download_flows_async(dst.array)
+        b.synchronize()
+        off = 0
+        for i, r in enumerate(rows):
+            P = r * (r * 4 // 3)
+            for k, w in enumerate(ref.flow(i)):
+                np.testing.assert_array_equal(dst.array[off + k * P: off + (k + 1) * P], w, err_msg=f"pair {i} plane {k} rep {rep}")
+            off += 3 * P
+        dst.array[:] = np.nan
+    src.free()
+    dst.free()
