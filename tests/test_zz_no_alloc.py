@@ -89,3 +89,48 @@ def test_noalloc_rejects_bad_arguments(gpu_available):
                                     bufs[0].ptr, None, bufs[2].ptr, bufs[3].ptr, None,
                                     outs[0].ptr, outs[1].ptr, outs[2].ptr, None)
     assert rc == 1 and lib.pdf_last_create_error().decode() == "null argument"
+
+
+def test_noalloc_solve_captured_into_a_graph(gpu_available):
+    """Code under HIP graph capture is one of the callers the entry point is for: its launches,
+    captured on a stream, form a graph whose every replay gives the eager call's flow (the outputs
+    are overwritten with NaN by eager copies between the replays)."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    vp = ctypes.c_void_p
+    for name, args in (("hipStreamCreate", [ctypes.POINTER(vp)]), ("hipStreamBeginCapture", [vp, ctypes.c_int]),
+                       ("hipStreamEndCapture", [vp, ctypes.POINTER(vp)]),
+                       ("hipGraphInstantiate", [ctypes.POINTER(vp), vp, vp, ctypes.c_char_p, ctypes.c_size_t]),
+                       ("hipGraphLaunch", [vp, vp]), ("hipStreamSynchronize", [vp]), ("hipGraphExecDestroy", [vp]),
+                       ("hipGraphDestroy", [vp]), ("hipStreamDestroy", [vp])):
+        getattr(hip, name).argtypes = args
+        getattr(hip, name).restype = ctypes.c_int
+    rows, cam_mode = 240, 2
+    say(f"allocation-free solve under graph capture rows={rows} cam_mode={cam_mode}")
+    bufs = upload_pair(7004, cam_mode)
+    want = noalloc_flow(bufs, rows, cam_mode)      # eager; also resolves the device array's address
+    n = rows * (rows * 4 // 3)
+    outs = [plib.DeviceBuffer(4 * n) for _ in range(3)]
+    stream, graph, exe = vp(), vp(), vp()
+    assert hip.hipStreamCreate(ctypes.byref(stream)) == 0
+    try:
+        assert hip.hipStreamBeginCapture(stream, 1) == 0        # hipStreamCaptureModeThreadLocal
+        try:
+            plib.solve_pair_noalloc(bufs, outs, rows=rows, cam_mode=cam_mode, stream=stream)
+        finally:
+            end = hip.hipStreamEndCapture(stream, ctypes.byref(graph))
+        assert end == 0 and graph.value
+        assert hip.hipGraphInstantiate(ctypes.byref(exe), graph, None, None, 0) == 0
+        for rep in range(3):
+            for o in outs:
+                o.upload(np.full(n, np.nan, np.float32))
+            assert hip.hipGraphLaunch(exe, stream) == 0
+            assert hip.hipStreamSynchronize(stream) == 0
+            got = [o.download(np.empty(n, np.float32)) for o in outs]
+            for name, a, b in zip(("dx", "dy", "dz"), got, want):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f"replay {rep}: {name} differs"
+    finally:
+        if exe.value:
+            hip.hipGraphExecDestroy(exe)
+        if graph.value:
+            hip.hipGraphDestroy(graph)
+        hip.hipStreamDestroy(stream)
