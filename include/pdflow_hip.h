@@ -231,15 +231,24 @@ int pdf_batch_set_iteration_kernel(pdf_batch *b, int k, unsigned min_pixels, int
 /* region shape of the temporally blocked kernel for the batch (as pdf_set_tb_shape) */
 int pdf_batch_set_tb_shape(pdf_batch *b, int shape);
 /* Autotuner: times the batch's own solve (one captured solve, then two graph replays between HIP
- * events) for each region shape and iterations per launch K in {3, 4, 5, 6}, keeps the fastest and
- * returns it (any output may be NULL; *ms = one solve).  Results are identical for every choice.
- * With cache_path, a text file of "<key>\t<shape> <K> <ms>" lines keyed by device and batch
- * composition: a hit applies the cached choice without timing, a miss appends the new line; lines
- * that do not parse to a supported shape and K are skipped.  A cache_path that exists but is not a
- * readable regular file (a directory, no read permission) applies the default (32 x 32, K = 4)
- * without timing and returns PDF_OK with *ms = 0.  The batch's inputs must be set (it is solved
- * several times). */
+ * events) for each region shape and iterations per launch K in {3, 4, 5, 6} with the batch's
+ * set-up tile, then each set-up tile (pdf_batch_set_setup_tile) with the fastest of those; keeps
+ * the fastest configuration and returns its shape, K and time (any output may be NULL; *ms = one
+ * solve; pdf_batch_setup_tile gives the tile).  Results are identical for every choice.  With
+ * cache_path, a text file of "<key>\t<shape> <K> <ms> <tile>" lines keyed by device and batch
+ * composition: a hit applies the cached choice without timing (a line without the tile applies the
+ * default tile), a miss appends the new line; lines that do not parse to a supported shape, K and
+ * tile are skipped.  A cache_path that exists but is not a readable regular file (a directory, no
+ * read permission) applies the defaults (32 x 32, K = 4, tile 32 x 8) without timing and returns
+ * PDF_OK with *ms = 0.  The batch's inputs must be set (it is solved several times). */
 int pdf_batch_autotune(pdf_batch *b, const char *cache_path, int *shape, int *k, float *ms);
+/* No counterpart in the reference (its kernels have one launch shape, pdflow_cudalib.h:28-29): the
+ * tile of the batch's set-up stencils that stage a neighbourhood through LDS, the level-init
+ * upsampling and the median filter.  Rows x columns, one pixel per thread: 0 = 32x8 (default),
+ * 1 = 16x16, 2 = 8x32 (4 waves), 3 = 32x4, 4 = 16x8, 5 = 8x16 (2 waves), 6 = 8x8 (1 wave).  Results
+ * do not depend on it; changing it invalidates the recorded plan. */
+int pdf_batch_set_setup_tile(pdf_batch *b, int tile);
+int pdf_batch_setup_tile(const pdf_batch *b, int *tile);
 /* device pointers of pair's four input planes (the caller may fill them directly).  They are
  * also level 0 of the pair's colour and depth pyramids, so new inputs take effect there at once:
  * a continuation or re-solve after new inputs would mix them with the pyramid levels the last

@@ -107,6 +107,7 @@ struct pdf_batch {
     bool own_stream = false;
     int tb_k = kDefaultTBK;
     int tb_shape = kDefaultTBShape;
+    int setup_tile = pdf::kDefaultSetupTile;   // tile of the LDS-staged set-up stencils
     unsigned tb_min_pixels = 60 * 80;
     bool level_kernel = true;
     // plan
@@ -350,7 +351,7 @@ void add_setup(pdf_batch *b, Plan &plan, pdf::BatchKernel k, int flag, const std
     for (unsigned i : ids) {
         const Pair &p = b->pairs[i];
         args.push_back(level_args(b, p));
-        blocks.push_back(pdf::batch_blocks(k, p.rows_i, p.cols_i));
+        blocks.push_back(pdf::batch_blocks(k, p.rows_i, p.cols_i, b->setup_tile));
     }
     add_launch(plan, setup_launch(k, flag), args, blocks);
 }
@@ -551,7 +552,7 @@ int run_plan(pdf_batch *b, const Plan &plan, char *dev_tables, bool timed)
         }
         hipError_t e;
         switch (l.kind) {
-        case Launch::Setup: e = pdf::launch_batch(l.setup, t, l.blocks, l.flag, b->stream); break;
+        case Launch::Setup: e = pdf::launch_batch(l.setup, t, l.blocks, l.flag, b->setup_tile, b->stream); break;
         case Launch::PD: e = pdf::launch_pd_batch(l.k, b->tb_shape, l.flag != 0, t, l.blocks, b->stream); break;
         default: e = pdf::launch_learn(l.learn, t, l.blocks, nullptr, b->stream); break;
         }
@@ -805,6 +806,24 @@ int pdf_batch_set_tb_shape(pdf_batch *b, int shape)
     return PDF_OK;
 }
 
+int pdf_batch_set_setup_tile(pdf_batch *b, int tile)
+{
+    if (!b) return PDF_ERR_ARG;
+    if (!pdf::setup_tile_supported(tile)) return bfail(b, PDF_ERR_ARG, "unsupported set-up tile");
+    if (tile != b->setup_tile) {
+        drop_plan(b);
+        b->setup_tile = tile;
+    }
+    return PDF_OK;
+}
+
+int pdf_batch_setup_tile(const pdf_batch *b, int *tile)
+{
+    if (!b || !tile) return PDF_ERR_ARG;
+    *tile = b->setup_tile;
+    return PDF_OK;
+}
+
 int pdf_batch_autotune(pdf_batch *b, const char *cache_path, int *shape_out, int *k_out, float *ms_out)
 {
     if (!b) return PDF_ERR_ARG;
@@ -818,6 +837,7 @@ int pdf_batch_autotune(pdf_batch *b, const char *cache_path, int *shape_out, int
         if (::stat(cache_path, &st) == 0 && (!S_ISREG(st.st_mode) || ::access(cache_path, R_OK) != 0)) {
             int rc = pdf_batch_set_tb_shape(b, kDefaultTBShape);
             if (!rc) rc = pdf_batch_set_iteration_kernel(b, kDefaultTBK, b->tb_min_pixels, b->level_kernel ? 1 : 0);
+            if (!rc) rc = pdf_batch_set_setup_tile(b, pdf::kDefaultSetupTile);
             if (rc) return rc;
             if (shape_out) *shape_out = kDefaultTBShape;
             if (k_out) *k_out = kDefaultTBK;
@@ -829,12 +849,15 @@ int pdf_batch_autotune(pdf_batch *b, const char *cache_path, int *shape_out, int
         while (std::getline(in, line)) {
             const size_t tab = line.find('\t');
             if (tab != std::string::npos && line.compare(0, tab, key) == 0) {
-                int shape = -1, k = -1;
+                int shape = -1, k = -1, tile = pdf::kDefaultSetupTile;
                 float ms = 0.f;
-                if (std::sscanf(line.c_str() + tab + 1, "%d %d %f", &shape, &k, &ms) == 3 &&
-                    pdf::pd_tb_shape_supported(shape) && pdf::pd_tb_supported(k)) {
+                // lines written before the set-up tile was tuned hold three fields
+                const int got = std::sscanf(line.c_str() + tab + 1, "%d %d %f %d", &shape, &k, &ms, &tile);
+                if (got >= 3 && pdf::pd_tb_shape_supported(shape) && pdf::pd_tb_supported(k) &&
+                    pdf::setup_tile_supported(tile)) {
                     int rc = pdf_batch_set_tb_shape(b, shape);
                     if (!rc) rc = pdf_batch_set_iteration_kernel(b, k, b->tb_min_pixels, b->level_kernel ? 1 : 0);
+                    if (!rc) rc = pdf_batch_set_setup_tile(b, tile);
                     if (rc) return rc;
                     if (shape_out) *shape_out = shape;
                     if (k_out) *k_out = k;
@@ -846,45 +869,58 @@ int pdf_batch_autotune(pdf_batch *b, const char *cache_path, int *shape_out, int
         }
     }
     // time every candidate: one captured solve to build the plan and graph, then two replays
-    // between HIP events on the batch's stream
+    // between HIP events on the batch's stream.  First the iteration kernel's region shape and K
+    // with the set-up tile the batch has, then the set-up tiles with the best of those.
     static const int kShapes[] = {pdf::kTBShape32x32, pdf::kTBShape64x32};
     static const int kKs[] = {3, 4, 5, 6};
-    int best_shape = b->tb_shape, best_k = b->tb_k;
-    float best = 1e30f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     BHIP(b, hipEventCreate(&e0));
     BHIP(b, hipEventCreate(&e1));
+    auto time_config = [&](int shape, int k, int tile, float *ms) -> int {
+        int rc = pdf_batch_set_tb_shape(b, shape);
+        if (!rc) rc = pdf_batch_set_iteration_kernel(b, k, b->tb_min_pixels, b->level_kernel ? 1 : 0);
+        if (!rc) rc = pdf_batch_set_setup_tile(b, tile);
+        if (!rc) rc = pdf_batch_solve(b, 1);
+        if (rc) return rc;
+        if (hipEventRecord(e0, b->stream) != hipSuccess) return bfail(b, PDF_ERR_HIP, "event record");
+        for (int r = 0; r < 2 && !rc; r++) rc = pdf_batch_solve(b, 1);
+        if (rc) return rc;
+        if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+            return bfail(b, PDF_ERR_HIP, "event timing");
+        *ms *= 0.5f;
+        return PDF_OK;
+    };
+    int best_shape = b->tb_shape, best_k = b->tb_k, best_tile = b->setup_tile;
+    float best = 1e30f;
     int rc = PDF_OK;
+    const int tile0 = b->setup_tile;
     for (int shape : kShapes) {
         for (int k : kKs) {
-            rc = pdf_batch_set_tb_shape(b, shape);
-            if (!rc) rc = pdf_batch_set_iteration_kernel(b, k, b->tb_min_pixels, b->level_kernel ? 1 : 0);
-            if (!rc) rc = pdf_batch_solve(b, 1);
-            if (rc) break;
-            if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = bfail(b, PDF_ERR_HIP, "event record"); break; }
-            for (int r = 0; r < 2 && !rc; r++) rc = pdf_batch_solve(b, 1);
-            if (rc) break;
             float ms = 0.f;
-            if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-                rc = bfail(b, PDF_ERR_HIP, "event timing");
-                break;
-            }
-            ms *= 0.5f;
+            rc = time_config(shape, k, tile0, &ms);
+            if (rc) break;
             if (ms < best) { best = ms; best_shape = shape; best_k = k; }
         }
         if (rc) break;
+    }
+    for (int tile = 0; !rc && tile < pdf::kNumSetupTiles; tile++) {
+        if (tile == tile0) continue;
+        float ms = 0.f;
+        rc = time_config(best_shape, best_k, tile, &ms);
+        if (!rc && ms < best) { best = ms; best_tile = tile; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     rc = pdf_batch_set_tb_shape(b, best_shape);
     if (!rc) rc = pdf_batch_set_iteration_kernel(b, best_k, b->tb_min_pixels, b->level_kernel ? 1 : 0);
+    if (!rc) rc = pdf_batch_set_setup_tile(b, best_tile);
     if (rc) return rc;
     if (cache_path) {
         std::ofstream out(cache_path, std::ios::trunc);
         for (const std::string &l : lines) out << l << "\n";
-        out << key << "\t" << best_shape << " " << best_k << " " << best << "\n";
+        out << key << "\t" << best_shape << " " << best_k << " " << best << " " << best_tile << "\n";
         if (!out) return bfail(b, PDF_ERR_ARG, std::string("cannot write ") + cache_path);
     }
     if (shape_out) *shape_out = best_shape;

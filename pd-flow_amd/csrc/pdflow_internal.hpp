@@ -219,12 +219,19 @@ hipError_t launch_warm_init(const LevelArgs &a, hipStream_t s);
 hipError_t launch_warp_mu(const LevelArgs &a, bool write_steps, hipStream_t s);
 hipError_t launch_filter_motion(const LevelArgs &a, hipStream_t s);
 hipError_t launch_pyramid2(const Pyr2Args &a, hipStream_t s);
-// batched launches (flag: has_coarse for LevelInit, unused otherwise)
-hipError_t launch_batch(BatchKernel k, const BatchTable &t, unsigned total_blocks, int flag, hipStream_t s);
+// tiles (rows x columns, one pixel per thread: 1 to 4 waves) of a batch's LDS-staged set-up
+// stencils, the level-init upsampling and the median filter: X(id, rows, columns)
+#define PDF_SETUP_TILES(X) X(0, 32, 8) X(1, 16, 16) X(2, 8, 32) X(3, 32, 4) X(4, 16, 8) X(5, 8, 16) X(6, 8, 8)
+constexpr int kNumSetupTiles = 7, kDefaultSetupTile = 0;
+bool setup_tile_supported(int tile);
+// batched launches (flag: has_coarse for LevelInit, unused otherwise; tile: the set-up tile of
+// LevelInit and FilterMotion)
+hipError_t launch_batch(BatchKernel k, const BatchTable &t, unsigned total_blocks, int flag, int tile,
+                        hipStream_t s);
 unsigned batch_blocks_per_level(unsigned pixels);
 // blocks a batched set-up kernel takes for one item of rows x cols (the 2-D tiled kernels
 // count tiles, the others 256-pixel blocks)
-unsigned batch_blocks(BatchKernel k, unsigned rows, unsigned cols);
+unsigned batch_blocks(BatchKernel k, unsigned rows, unsigned cols, int tile);
 // blocks of one frame's pyramid level in a batched pyramid launch
 unsigned batch_pyramid_blocks(unsigned level, unsigned rows, unsigned cols);
 // k = 0: whole-level kernel (iters from PDArgs), 1: one iteration, >1: temporal blocking;

@@ -34,28 +34,33 @@ __constant__ int kCarry[10] = {PDF_DU_L, PDF_DV_L, PDF_DW_L, PDF_PD_L, PDF_PUU_L
 __constant__ int kUpDst[10] = {PDF_DU_PREV, PDF_DV_PREV, PDF_DW_NEW, PDF_PD, PDF_PUU,
                                PDF_PUV, PDF_PVU, PDF_PVV, PDF_PWU, PDF_PWV};
 
-// 2-D tiles of 32 rows x 8 columns (256 threads, lanes down a column) for the set-up kernels
-// that read a neighbourhood: the neighbourhood is staged through LDS once per tile.
+// 2-D tiles of TR rows x TC columns (one pixel per thread, lanes down a column) for the set-up
+// kernels that read a neighbourhood: the neighbourhood is staged through LDS once per tile.  The
+// single-pair kernels and the pyramid use 32 x 8 (256 threads); a batch's level-init and median
+// filter take any tile of PDF_SETUP_TILES (pdf_batch_set_setup_tile), results unchanged.
 constexpr int kTR = 32, kTC = 8;
+template <int TR = kTR, int TC = kTC>
 __host__ __device__ inline unsigned tiles_of(unsigned rows, unsigned cols)
 {
-    return ((rows + kTR - 1) / kTR) * ((cols + kTC - 1) / kTC);
+    return ((rows + TR - 1) / TR) * ((cols + TC - 1) / TC);
 }
 
-// the coarse patch a tile's 5x5 even taps touch: coarse rows v0/2-1 .. v0/2+16, columns
-// u0/2-1 .. u0/2+4 (clamped to the coarse level), at most 18 x 6, stored with row stride kCR;
-// element kZero of each plane holds +0 for the taps that fall outside the level
-constexpr int kCR = kTR / 2 + 3, kCC = kTC / 2 + 3, kZero = kCR * kCC;
-static_assert(kCR * kCC <= kBlock, "one coarse element per thread and plane");
-
+// the coarse patch a tile's 5x5 even taps touch: coarse rows v0/2-1 .. v0/2+TR/2, columns
+// u0/2-1 .. u0/2+TC/2 (clamped to the coarse level), at most (TR/2+2) x (TC/2+2), stored with
+// row stride CR = TR/2+3; element CR*CC of each plane holds +0 for the taps that fall outside
+// the level
+template <int TR = kTR, int TC = kTC>
 __device__ __forceinline__ void level_init_body(const LevelArgs &a, unsigned tile, int has_coarse,
                                                 bool write_const = true)
 {
+    constexpr int kCR = TR / 2 + 3, kCC = TC / 2 + 3, kZero = kCR * kCC;
+    static_assert(kCR * kCC <= TR * TC, "one coarse element per thread and plane");
+    static_assert(TR % 2 == 0 && TC % 2 == 0, "tiles start at even rows and columns");
     __shared__ float sC[10][kZero + 1];
     const unsigned rows_i = a.rows, cols_i = a.cols;
-    const unsigned tv = (rows_i + kTR - 1) / kTR;
-    const int v0 = (int)(tile % tv) * kTR, u0 = (int)(tile / tv) * kTC;
-    const int v = v0 + (int)threadIdx.x % kTR, u = u0 + (int)threadIdx.x / kTR;
+    const unsigned tv = (rows_i + TR - 1) / TR;
+    const int v0 = (int)(tile % tv) * TR, u0 = (int)(tile / tv) * TC;
+    const int v = v0 + (int)threadIdx.x % TR, u = u0 + (int)threadIdx.x / TR;
     const bool on = v < (int)rows_i && u < (int)cols_i;
     const unsigned index = on ? (unsigned)v + (unsigned)u * rows_i : 0u;
     float acc[10];
@@ -63,8 +68,8 @@ __device__ __forceinline__ void level_init_body(const LevelArgs &a, unsigned til
     for (int q = 0; q < 10; q++) acc[q] = 0.f;
     if (has_coarse) {
         const int rc = (int)rows_i / 2, cc = (int)cols_i / 2;
-        const int cr0 = v0 >= 2 ? (v0 - 2) / 2 : 0, cr1 = min(rc - 1, (v0 + kTR + 1) / 2);
-        const int cc0 = u0 >= 2 ? (u0 - 2) / 2 : 0, cc1 = min(cc - 1, (u0 + kTC + 1) / 2);
+        const int cr0 = v0 >= 2 ? (v0 - 2) / 2 : 0, cr1 = min(rc - 1, (v0 + TR + 1) / 2);
+        const int cc0 = u0 >= 2 ? (u0 - 2) / 2 : 0, cc1 = min(cc - 1, (u0 + TC + 1) / 2);
         const int t = (int)threadIdx.x, tr = t % kCR, tc = t / kCR;
         if (t < kCR * kCC && tr <= cr1 - cr0 && tc <= cc1 - cc0) {
             const unsigned src = (unsigned)(cr0 + tr) + (unsigned)(cc0 + tc) * (unsigned)rc;
@@ -304,16 +309,17 @@ __device__ __forceinline__ Sat saturated(const LevelArgs &a, unsigned i)
 
 // carry_duals: copy pd and the six duals into the *_l carry planes that the next level's
 // upsampling reads (.cu:1134-1147); a pair's finest level has no next level
+template <int TR = kTR, int TC = kTC>
 __device__ __forceinline__ void filter_motion_body(const LevelArgs &a, unsigned tile, bool carry_duals = true)
 {
     // the tile plus a one-pixel halo: saturated solution, frame-1 depth and ddt of each pixel,
     // computed once per pixel and read by the (up to) 9 pixels whose window contains it
-    constexpr int HR = kTR + 2, HC = kTC + 2;
+    constexpr int HR = TR + 2, HC = TC + 2;
     __shared__ float sU[HR * HC], sV[HR * HC], sW[HR * HC], sZ[HR * HC], sT[HR * HC];
     const unsigned rows_i = a.rows, cols_i = a.cols;
-    const unsigned tv = (rows_i + kTR - 1) / kTR;
-    const int v0 = (int)(tile % tv) * kTR, u0 = (int)(tile / tv) * kTC;
-    for (int e = (int)threadIdx.x; e < HR * HC; e += kBlock) {
+    const unsigned tv = (rows_i + TR - 1) / TR;
+    const int v0 = (int)(tile % tv) * TR, u0 = (int)(tile / tv) * TC;
+    for (int e = (int)threadIdx.x; e < HR * HC; e += TR * TC) {
         const int vv = v0 - 1 + e % HR, uu = u0 - 1 + e / HR;
         if (vv >= 0 && vv < (int)rows_i && uu >= 0 && uu < (int)cols_i) {
             const unsigned il = (unsigned)vv + (unsigned)uu * rows_i;
@@ -326,7 +332,7 @@ __device__ __forceinline__ void filter_motion_body(const LevelArgs &a, unsigned 
         }
     }
     __syncthreads();
-    const int lv = (int)threadIdx.x % kTR, lu = (int)threadIdx.x / kTR;
+    const int lv = (int)threadIdx.x % TR, lu = (int)threadIdx.x / TR;
     const int v = v0 + lv, u = u0 + lu;
     if (v >= (int)rows_i || u >= (int)cols_i) return;
     const unsigned index = (unsigned)v + (unsigned)u * rows_i;
@@ -559,11 +565,12 @@ __global__ void __launch_bounds__(kBlock) k_warp_mu(LevelArgs a, int write_steps
 __global__ void __launch_bounds__(kBlock) k_filter_motion(LevelArgs a) { filter_motion_body(a, blockIdx.x); }
 __global__ void __launch_bounds__(kBlock) k_pyramid2(Pyr2Args a) { pyramid_body<1>(a.frame[blockIdx.y], blockIdx.x); }
 
-__global__ void __launch_bounds__(kBlock) kb_level_init(BatchTable t, int flag)
+template <int TR, int TC>
+__global__ void __launch_bounds__(TR * TC) kb_level_init(BatchTable t, int flag)
 {
     const BatchItem it = setup_item(t);
-    level_init_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, flag & kInitCoarse,
-                    (flag & kInitSkipConst) == 0);
+    level_init_body<TR, TC>(static_cast<const LevelArgs *>(t.args)[it.item], it.block, flag & kInitCoarse,
+                            (flag & kInitSkipConst) == 0);
 }
 __global__ void __launch_bounds__(kBlock) kb_warm_init(BatchTable t)
 {
@@ -581,10 +588,12 @@ __global__ void __launch_bounds__(kBlock) kb_warp_mu(BatchTable t)
     warp_mu_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, 0);
 }
 // flag kFilterNoCarry: the duals are not copied into the *_l planes
-__global__ void __launch_bounds__(kBlock) kb_filter_motion(BatchTable t, int flag)
+template <int TR, int TC>
+__global__ void __launch_bounds__(TR * TC) kb_filter_motion(BatchTable t, int flag)
 {
     const BatchItem it = setup_item(t);
-    filter_motion_body(static_cast<const LevelArgs *>(t.args)[it.item], it.block, (flag & kFilterNoCarry) == 0);
+    filter_motion_body<TR, TC>(static_cast<const LevelArgs *>(t.args)[it.item], it.block,
+                               (flag & kFilterNoCarry) == 0);
 }
 __global__ void __launch_bounds__(kBlock) kb_pyramid(BatchTable t)
 {
@@ -644,16 +653,34 @@ hipError_t launch_pyramid2(const Pyr2Args &a, hipStream_t s)
 
 namespace pdf {
 
-hipError_t launch_batch(BatchKernel k, const BatchTable &t, unsigned total_blocks, int flag,
+bool setup_tile_supported(int tile) { return tile >= 0 && tile < kNumSetupTiles; }
+
+hipError_t launch_batch(BatchKernel k, const BatchTable &t, unsigned total_blocks, int flag, int tile,
                         hipStream_t s)
 {
     if (total_blocks == 0 || t.n == 0) return hipSuccess;
     const dim3 grid(total_blocks), block(kBlock);
     switch (k) {
-    case BatchKernel::LevelInit: hipLaunchKernelGGL(kb_level_init, grid, block, 0, s, t, flag); break;
+    case BatchKernel::LevelInit:
+        switch (tile) {
+#define PDF_LAUNCH_TILE(id, R, C) \
+        case id: hipLaunchKernelGGL((kb_level_init<R, C>), grid, dim3(R * C), 0, s, t, flag); break;
+        PDF_SETUP_TILES(PDF_LAUNCH_TILE)
+#undef PDF_LAUNCH_TILE
+        default: return hipErrorInvalidValue;
+        }
+        break;
     case BatchKernel::RijGradients: hipLaunchKernelGGL(kb_rij_gradients, grid, block, 0, s, t); break;
     case BatchKernel::WarpMu: hipLaunchKernelGGL(kb_warp_mu, grid, block, 0, s, t); break;
-    case BatchKernel::FilterMotion: hipLaunchKernelGGL(kb_filter_motion, grid, block, 0, s, t, flag); break;
+    case BatchKernel::FilterMotion:
+        switch (tile) {
+#define PDF_LAUNCH_TILE(id, R, C) \
+        case id: hipLaunchKernelGGL((kb_filter_motion<R, C>), grid, dim3(R * C), 0, s, t, flag); break;
+        PDF_SETUP_TILES(PDF_LAUNCH_TILE)
+#undef PDF_LAUNCH_TILE
+        default: return hipErrorInvalidValue;
+        }
+        break;
     case BatchKernel::Pyramid: hipLaunchKernelGGL(kb_pyramid, grid, block, 0, s, t); break;
     case BatchKernel::WarmInit: hipLaunchKernelGGL(kb_warm_init, grid, block, 0, s, t); break;
     default: return hipErrorInvalidValue;
@@ -669,10 +696,18 @@ unsigned batch_pyramid_blocks(unsigned level, unsigned rows, unsigned cols)
     return tiles_of(rows, cols);
 }
 
-unsigned batch_blocks(BatchKernel k, unsigned rows, unsigned cols)
+unsigned batch_blocks(BatchKernel k, unsigned rows, unsigned cols, int tile)
 {
-    if (k == BatchKernel::LevelInit || k == BatchKernel::FilterMotion || k == BatchKernel::Pyramid)
-        return tiles_of(rows, cols);
+    if (k == BatchKernel::LevelInit || k == BatchKernel::FilterMotion) {
+        switch (tile) {
+#define PDF_TILE_COUNT(id, R, C) \
+        case id: return tiles_of<R, C>(rows, cols);
+        PDF_SETUP_TILES(PDF_TILE_COUNT)
+#undef PDF_TILE_COUNT
+        default: return 0;
+        }
+    }
+    if (k == BatchKernel::Pyramid) return tiles_of(rows, cols);
     return batch_blocks_per_level(rows * cols);
 }
 

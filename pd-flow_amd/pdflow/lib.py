@@ -119,6 +119,8 @@ def hip_library():
         "pdf_batch_flow_device": ([vp, c_uint, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), up_, up_], c_int),
         "pdf_batch_copy_flow": ([vp, c_uint, fp, fp, fp], c_int),
         "pdf_batch_upload_inputs_async": ([vp, fp, ctypes.c_size_t], c_int),
+        "pdf_batch_set_setup_tile": ([vp, c_int], c_int),
+        "pdf_batch_setup_tile": ([vp, ctypes.POINTER(c_int)], c_int),
         "pdf_batch_download_flows_async": ([vp, fp, ctypes.c_size_t], c_int),
         "pdf_batch_launch_count": ([vp, up_], c_int),
         "pdf_batch_set_start_level": ([vp, c_uint, c_uint, fpp], c_int),
@@ -471,9 +473,22 @@ class Batch:
     def set_tb_shape(self, shape):
         self._chk(self.lib.pdf_batch_set_tb_shape(self.ctx, shape))
 
+    SETUP_TILES = ((32, 8), (16, 16), (8, 32), (32, 4), (16, 8), (8, 16), (8, 8))
+
+    def set_setup_tile(self, tile):
+        """tile of the LDS-staged set-up stencils (level-init, median filter): an index into
+        SETUP_TILES (rows x columns, one pixel per thread)"""
+        self._chk(self.lib.pdf_batch_set_setup_tile(self.ctx, tile))
+
+    def setup_tile(self):
+        t = ctypes.c_int()
+        self._chk(self.lib.pdf_batch_setup_tile(self.ctx, ctypes.byref(t)))
+        return t.value
+
     def autotune(self, cache_path=None):
-        """picks the fastest region shape and iterations per launch for this batch (see
-        pdf_batch_autotune); returns (shape, k, ms per solve)"""
+        """picks the fastest region shape and iterations per launch for this batch, then the
+        fastest set-up tile (see pdf_batch_autotune; setup_tile() gives it); returns (shape, k,
+        ms per solve)"""
         shape, k, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
         self._chk(self.lib.pdf_batch_autotune(self.ctx, str(cache_path).encode() if cache_path else None,
                                               ctypes.byref(shape), ctypes.byref(k), ctypes.byref(ms)))

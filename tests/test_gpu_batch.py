@@ -157,19 +157,25 @@ def test_autotune_picks_a_config_and_caches_it(gpu_available, tmp_path):
         b.set_inputs(i, *inputs)
         want.append(flow)
     shape, k, ms = b.autotune(cache)
-    assert shape in (0, 1) and k in (3, 4, 5, 6) and ms > 0
+    tile = b.setup_tile()
+    assert shape in (0, 1) and k in (3, 4, 5, 6) and ms > 0 and 0 <= tile < len(Batch.SETUP_TILES)
     b.solve(graph=True)
     b.synchronize()
     for i in range(len(rows)):
         for a, w in zip(b.flow(i), want[i]):
             np.testing.assert_array_equal(a, w)
     lines = cache.read_text().splitlines()
-    assert len(lines) == 1 and lines[0].split("\t")[1].split()[:2] == [str(shape), str(k)]
+    fields = lines[0].split("\t")[1].split()
+    assert len(lines) == 1 and fields[:2] == [str(shape), str(k)] and fields[3] == str(tile)
     b2 = Batch(rows)
     for i, r in enumerate(rows):
         b2.set_inputs(i, *single(r, 900 + i)[0])
-    assert b2.autotune(cache)[:2] == (shape, k)
+    assert b2.autotune(cache)[:2] == (shape, k) and b2.setup_tile() == tile
     assert len(cache.read_text().splitlines()) == 1
+    # a line written before the set-up tile was tuned (three fields) applies the default tile
+    cache.write_text(lines[0].split("\t")[0] + "\t" + " ".join(fields[:3]) + "\n")
+    b2.set_setup_tile(6)
+    assert b2.autotune(cache)[:2] == (shape, k) and b2.setup_tile() == 0
 
 
 def test_autotune_falls_back_to_the_default_tile(gpu_available, tmp_path):
@@ -262,3 +268,26 @@ def test_async_inputs_and_flows_match_the_blocking_copies(gpu_available):
         dst.array[:] = np.nan
     src.free()
     dst.free()
+
+
+def test_every_setup_tile_matches_single_pairs(gpu_available):
+    """The level-init and median-filter stencils of a batch give the same bits for every set-up
+    tile (32x8 .. 8x8 pixels, 1 to 4 waves), on a batch whose levels are not multiples of the
+    tiles (15x20, 30x40, ... 480x640)."""
+    rows = [480, 240, 120, 60, 30, 15, 240]
+    want, b = [], Batch(rows)
+    for i, r in enumerate(rows):
+        inputs, flow = single(r, 1300 + i)
+        b.set_inputs(i, *inputs)
+        want.append(flow)
+    with pytest.raises(PDFlowError):
+        b.set_setup_tile(len(Batch.SETUP_TILES))
+    for tile, (tr, tc) in enumerate(Batch.SETUP_TILES):
+        say(f"set-up tile {tr}x{tc}")
+        b.set_setup_tile(tile)
+        assert b.setup_tile() == tile
+        b.solve(graph=tile % 2 == 0)
+        b.synchronize()
+        for i in range(len(rows)):
+            for a, w in zip(b.flow(i), want[i]):
+                np.testing.assert_array_equal(a, w, err_msg=f"tile {tr}x{tc}, pair {i}")
