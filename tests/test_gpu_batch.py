@@ -291,3 +291,33 @@ def test_every_setup_tile_matches_single_pairs(gpu_available):
         for i in range(len(rows)):
             for a, w in zip(b.flow(i), want[i]):
                 np.testing.assert_array_equal(a, w, err_msg=f"tile {tr}x{tc}, pair {i}")
+
+
+def test_async_copies_when_frames_are_larger_than_the_solve(gpu_available):
+    """Frames larger than the solved level (cam_mode below 480 / rows): the asynchronous upload
+    takes each pair's frames at their own size and the download gives the flow at the solved
+    size, equal to the blocking copies."""
+    from pdflow import synth_pair_float
+    rows, cams = [120, 240, 120], [2, 1, 4]
+    say(f"async copies, rows {rows}, cam_modes {cams}")
+    frames = [synth_pair_float(1400 + i, 480 // c, 640 // c) for i, c in enumerate(cams)]
+    b, ref = Batch(rows, cam_modes=cams), Batch(rows, cam_modes=cams)
+    for i, f in enumerate(frames):
+        ref.set_inputs(i, *f)
+    ref.solve(graph=True)
+    ref.synchronize()
+    src = PinnedBuffer(sum(4 * (480 // c) * (640 // c) for c in cams))
+    dst = PinnedBuffer(sum(3 * r * (r * 4 // 3) for r in rows))
+    src.array[:] = np.concatenate([np.ascontiguousarray(x, np.float32).ravel() for f in frames for x in f])
+    b.upload_inputs_async(src.array)
+    b.solve(graph=True)
+    b.download_flows_async(dst.array)
+    b.synchronize()
+    off = 0
+    for i, r in enumerate(rows):
+        P = r * (r * 4 // 3)
+        for k, w in enumerate(ref.flow(i)):
+            np.testing.assert_array_equal(dst.array[off + k * P: off + (k + 1) * P], w, err_msg=f"pair {i} plane {k}")
+        off += 3 * P
+    src.free()
+    dst.free()
