@@ -72,6 +72,18 @@ def measured_traffic(workload_key):
     return None, "no traffic profile of this workload"
 
 
+def measured_l2_hit_rate(traffic_source):
+    """the L2 hit rate of the same launches, from the 'L2 hit rate over the' line of the traffic
+    profile that gave the traffic (None without one)"""
+    if not traffic_source or not traffic_source.startswith("profiles/"):
+        return None
+    with open(os.path.join(ROOT, traffic_source)) as f:
+        for line in f:
+            if line.startswith("L2 hit rate over the"):
+                return float(line.rsplit(":", 1)[1])
+    return None
+
+
 def roofline(kernel, timing, workload_key=None):
     """roofline of the dominant kernel from HIP-event timing (launches, ms, pixel-iterations,
     pixels covered) on the stream the launches run on"""
@@ -82,6 +94,7 @@ def roofline(kernel, timing, workload_key=None):
     return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": source,
+            "l2_hit_rate": measured_l2_hit_rate(source),
             "algorithmic_bytes_per_launch": round(BYTES_PER_PIXEL_LAUNCH * pixels / launches),
             "avg_launch_us": round(kms / launches * 1e3, 3), "launches": launches,
             "bytes_per_pixel_per_launch": BYTES_PER_PIXEL_LAUNCH,

@@ -11,6 +11,9 @@ bench.py reads back.
 ``trace``: the average duration of those launches in a ``--kernel-trace`` CSV of the same command,
 to set beside bench.py's HIP-event average.
 
+``l2``: the L2 hit rate of those launches from one ``--pmc TCC_HIT_sum TCC_MISS_sum`` run of the
+same command.
+
 ``kernels``: every dispatch of a ``--kernel-trace`` CSV grouped by kernel and grid, with totals
 divided by ``--per`` (the number of solves or pairs traced), and the average of the last
 ``--iteration-launches`` dispatches of the single-pair iteration kernels.
@@ -82,6 +85,32 @@ def traffic(a):
     print(f"traffic: bytes_per_launch={per:.6e} workload={a.workload} iterate_source_sha16={sha}")
 
 
+def l2(a):
+    """L2 hit rate of the last --launches kb_pd_tb dispatches from one --pmc TCC_HIT_sum TCC_MISS_sum run"""
+    rows = _sorted_dispatches(_rows(a.counters))
+    by = collections.defaultdict(dict)
+    for r in rows:
+        by[int(r["Dispatch_Id"])][r["Counter_Name"]] = float(r["Counter_Value"])
+        by[int(r["Dispatch_Id"])]["name"] = r["Kernel_Name"]
+        by[int(r["Dispatch_Id"])]["grid"] = int(r["Grid_Size"])
+    tb = [by[d] for d in sorted(by) if "kb_pd_tb" in by[d]["name"]][-a.launches:]
+    if len(tb) != a.launches:
+        raise SystemExit("fewer kb_pd_tb dispatches than --launches")
+    kinds = collections.OrderedDict()
+    for d in tb:
+        k = (d["name"].split("kb_pd_tb")[1].split("(")[0], d["grid"])
+        kinds.setdefault(k, []).append((d["TCC_HIT_sum"], d["TCC_MISS_sum"]))
+    print("| instance | grid (threads) | launches | L2 hits | L2 misses | hit rate |")
+    print("|---|---|---|---|---|---|")
+    th = tm = 0.0
+    for (name, grid), v in kinds.items():
+        h, m = sum(x for x, _ in v), sum(y for _, y in v)
+        th, tm = th + h, tm + m
+        print(f"| kb_pd_tb{name} | {grid} | {len(v)} | {h / len(v):,.0f} | {m / len(v):,.0f} | {h / (h + m):.3f} |")
+    print()
+    print(f"L2 hit rate over the {a.launches} launches: {th / (th + tm):.3f}")
+
+
 def trace(a):
     rows = sorted(_rows(a.trace), key=lambda r: int(r["Start_Timestamp"]))
     tb = [r for r in rows if "kb_pd_tb" in r["Kernel_Name"]][-a.launches:]
@@ -143,6 +172,9 @@ def kernels(a):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
+    h = sub.add_parser("l2")
+    h.add_argument("counters")
+    h.add_argument("--launches", type=int, required=True)
     g = sub.add_parser("kernels")
     g.add_argument("trace")
     g.add_argument("--per", type=float, default=1.0, help="divide totals by this count (solves, pairs)")
@@ -158,7 +190,7 @@ def main(argv=None):
     r.add_argument("trace")
     r.add_argument("--launches", type=int, required=True)
     a = ap.parse_args(argv)
-    {"traffic": traffic, "trace": trace, "kernels": kernels}[a.cmd](a)
+    {"traffic": traffic, "trace": trace, "kernels": kernels, "l2": l2}[a.cmd](a)
 
 
 if __name__ == "__main__":
