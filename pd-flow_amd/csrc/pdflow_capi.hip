@@ -660,12 +660,17 @@ int pdf_motion_field_device(pdf_ctx *c, const float **dx, const float **dy, cons
 
 static int fused_level(pdf_ctx *c, unsigned i, const unsigned *iters, const float *warm = nullptr)
 {
-    // the same stages as the bridges below, in four set-up launches (pdflow_setup.hip)
+    // the same stages as the bridges below, in three set-up launches (pdflow_setup.hip; four
+    // with a warm start)
     pdf::LevelArgs a = level_args(c);
     a.warm = warm;
-    int rc = warm ? check_launch(c, pdf::launch_warm_init(a, c->stream), "warm start")
-                  : check_launch(c, pdf::launch_level_init(a, i > 0, c->stream), "level init");
-    if (!rc) rc = check_launch(c, pdf::launch_rij_gradients(a, c->stream), "rij + gradients");
+    int rc;
+    if (warm) {
+        rc = check_launch(c, pdf::launch_warm_init(a, c->stream), "warm start");
+        if (!rc) rc = check_launch(c, pdf::launch_rij_gradients(a, c->stream), "rij + gradients");
+    } else {
+        rc = check_launch(c, pdf::launch_level_init_rij(a, i > 0, c->stream), "level init + rij + gradients");
+    }
     if (!rc) rc = check_launch(c, pdf::launch_warp_mu(a, false, c->stream), "warp + mu");
     if (!rc) rc = pdf_pd_iterations(c, iters[i]);
     if (!rc) rc = check_launch(c, pdf::launch_filter_motion(level_args(c), c->stream), "filter + motion");

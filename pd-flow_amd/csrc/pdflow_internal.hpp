@@ -215,6 +215,8 @@ hipError_t launch_filter(const LevelArgs &a, hipStream_t s);
 // fused set-up of the solve path (pdflow_setup.hip)
 hipError_t launch_level_init(const LevelArgs &a, bool has_coarse, hipStream_t s);
 hipError_t launch_rij_gradients(const LevelArgs &a, hipStream_t s);
+// launch_level_init and launch_rij_gradients of one level in a single launch
+hipError_t launch_level_init_rij(const LevelArgs &a, bool has_coarse, hipStream_t s);
 hipError_t launch_warm_init(const LevelArgs &a, hipStream_t s);
 hipError_t launch_warp_mu(const LevelArgs &a, bool write_steps, hipStream_t s);
 hipError_t launch_filter_motion(const LevelArgs &a, hipStream_t s);

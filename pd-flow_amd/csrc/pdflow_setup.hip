@@ -560,6 +560,15 @@ __device__ __forceinline__ BatchItem setup_item(const BatchTable &t)
 // its pair's arguments in a device table (see BatchTable).
 __global__ void __launch_bounds__(kBlock) k_level_init(LevelArgs a, int has_coarse) { level_init_body(a, blockIdx.x, has_coarse); }
 __global__ void __launch_bounds__(kBlock) k_rij_gradients(LevelArgs a) { rij_gradients_body(a, blockIdx.x); }
+// a level's level-init tiles and rij/gradients blocks in one launch: they read and write disjoint
+// planes, and a single pair's small levels pay about a launch's latency for each kernel
+__global__ void __launch_bounds__(kBlock) k_level_init_rij(LevelArgs a, int has_coarse, unsigned init_tiles)
+{
+    if (blockIdx.x < init_tiles)
+        level_init_body(a, blockIdx.x, has_coarse);
+    else
+        rij_gradients_body(a, blockIdx.x - init_tiles);
+}
 __global__ void __launch_bounds__(kBlock) k_warm_init(LevelArgs a) { warm_init_body(a, blockIdx.x); }
 __global__ void __launch_bounds__(kBlock) k_warp_mu(LevelArgs a, int write_steps) { warp_mu_body(a, blockIdx.x, write_steps); }
 __global__ void __launch_bounds__(kBlock) k_filter_motion(LevelArgs a) { filter_motion_body(a, blockIdx.x); }
@@ -615,6 +624,16 @@ hipError_t launch_warm_init(const LevelArgs &a, hipStream_t s)
     const unsigned n = a.rows * a.cols;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_warm_init, grid_for(n), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_level_init_rij(const LevelArgs &a, bool has_coarse, hipStream_t s)
+{
+    const unsigned n = a.rows * a.cols;
+    if (n == 0) return hipSuccess;
+    const unsigned tiles = tiles_of(a.rows, a.cols);
+    hipLaunchKernelGGL(k_level_init_rij, dim3(tiles + grid_for(n).x), dim3(kBlock), 0, s, a, has_coarse ? 1 : 0,
+                       tiles);
     return hipGetLastError();
 }
 
