@@ -39,14 +39,18 @@ HBM_PEAK_GBS = 8000.0
 # above the HBM peak; it is reported under its own name.
 BYTES_PER_PIXEL_LAUNCH = 148
 PROFILES = os.path.join(ROOT, "profiles")
-ITERATE_SRC = os.path.join(ROOT, "pd-flow_amd", "csrc", "pdflow_iterate.hip")
+# the sources of the iteration kernels (the kernels and the per-pixel arithmetic they inline)
+ITERATE_SRCS = [os.path.join(ROOT, "pd-flow_amd", "csrc", f) for f in ("pdflow_iterate.hip", "pdflow_pixel.hpp")]
 RULE_JSON = os.path.join(ROOT, "pd-flow_amd", "pdflow", "weights", "warmstart_qvga_rule.json")
 
 
-def source_sha16(path):
+def source_sha16(paths):
     import hashlib
-    with open(path, "rb") as f:
-        return hashlib.sha256(f.read()).hexdigest()[:16]
+    h = hashlib.sha256()
+    for path in paths:
+        with open(path, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
 
 
 def measured_traffic(workload_key):
@@ -54,7 +58,7 @@ def measured_traffic(workload_key):
     rocprofv3 --pmc summary (profiles/*_hbm_traffic.txt): its 'traffic:' line must name this
     workload and the iteration kernels' current source hash, else the reading is stale and None
     is returned with the reason"""
-    sha = source_sha16(ITERATE_SRC)
+    sha = source_sha16(ITERATE_SRCS)
     if not os.path.isdir(PROFILES):
         return None, "no profiles/ directory"
     for name in sorted(os.listdir(PROFILES)):

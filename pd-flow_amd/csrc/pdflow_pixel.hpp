@@ -88,13 +88,13 @@ __device__ __forceinline__ void project(float &a, float &b)
     // unit-disc projection (.cu:849-859): m = 1/sqrt(a*a+b*b), scale when m < 1.  With
     // correctly rounded sqrt and division, x = a*a+b*b <= 1 gives sqrt(x) <= 1 and so m >= 1,
     // and a NaN x gives a NaN m: neither scales.  Here x is raised to 1 first (fmaxf returns 1
-    // for a NaN x), which gives m = 1 in exactly those cases, and the reference's m for every
-    // x > 1, +inf included.
+    // for a NaN x), which gives m = 1 exactly in those cases, and the reference's m for every
+    // x > 1, +inf included.  So the product by m can be unconditional: a * 1 is a, -0 and
+    // infinities included (a NaN stays a NaN).
     const float x = a * a + b * b;
     const float m = inv_sqrt_rn_ge1(fmaxf(x, 1.f));
-    const bool scale = m < 1.f;
-    a = scale ? a * m : a;
-    b = scale ? b * m : b;
+    a = a * m;
+    b = b * m;
 }
 
 // project() for a pair that seldom leaves the unit disc (the depth-flow duals pwu, pwv: about 1%

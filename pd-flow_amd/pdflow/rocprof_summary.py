@@ -29,8 +29,9 @@ import csv
 import hashlib
 import os
 
-ITERATE_SRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "csrc",
-                           "pdflow_iterate.hip")
+# the sources of the iteration kernels (the kernels and the per-pixel arithmetic they inline)
+ITERATE_SRCS = [os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "csrc", f)
+                for f in ("pdflow_iterate.hip", "pdflow_pixel.hpp")]
 
 
 def _rows(path):
@@ -79,8 +80,11 @@ def traffic(a):
     per = tot / a.launches
     print()
     print(f"Average over the {a.launches} launches: {per / 1e9:.4f} GB of HBM traffic per launch.")
-    with open(ITERATE_SRC, "rb") as f:
-        sha = hashlib.sha256(f.read()).hexdigest()[:16]
+    h = hashlib.sha256()
+    for path in ITERATE_SRCS:
+        with open(path, "rb") as f:
+            h.update(f.read())
+    sha = h.hexdigest()[:16]
     print()
     print(f"traffic: bytes_per_launch={per:.6e} workload={a.workload} iterate_source_sha16={sha}")
 
