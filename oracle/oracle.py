@@ -22,7 +22,9 @@ def tanf(x):
 
 
 def lib_path():
-    return os.path.join(_HERE, "liboracle.so")
+    # PDF_ORACLE_LIB: another build of the oracle (tests/test_oracle_mutations.py loads builds
+    # with planted mistakes to show that the known-answer tests reject them)
+    return os.environ.get("PDF_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 
 def load():

@@ -454,6 +454,38 @@ def test_weighted_median_uniform_is_mean_of_middle_pair():
     assert mu == np.float32(4.5)          # mean of the 4th and 5th smallest, not the 5th
 
 
+@pytest.mark.parametrize("via", ["depth", "ddt"])
+def test_median_filter_weights(via):
+    """The median filter's tap weight is 1 / (1 + 5 dz^2 + 10 dt^2) with dz the depth difference to
+    the centre and dt the tap's ddt (.cu:1075).  Four taps of value 0 at the centre's depth and five
+    of value 1 at dz = 0.2 (or at dt = 0.1) move the half-weight point into the first 1 by a
+    fraction fixed by those weights: (4 + 5 w) / 2 - 4 = w * m with w = 1/1.2 (1/1.1) gives m = 0.1
+    (0.3), worked out by hand."""
+    o = _level_after_setup()
+    rows, cols = o.level_dims()
+    P = rows * cols
+    v0, u0 = 7, 10
+    depth = np.ones(P, np.float32)
+    ddt = np.zeros(P, np.float32)
+    du = np.zeros(P, np.float32)
+    ones = [(-1, -1), (0, -1), (1, 0), (1, 1), (-1, 1)]       # five taps; the other four stay 0
+    for k, l in ones:
+        i = (v0 + k) + (u0 + l) * rows
+        du[i] = 1.0
+        if via == "depth":
+            depth[i] = 1.2
+        else:
+            ddt[i] = 0.1
+    o.set_pyramid("depth_old", 0, depth)
+    for name, val in (("du_new", du), ("dv_new", du), ("dw_new", du), ("ddt", ddt),
+                      ("du_prev", np.zeros(P, np.float32)), ("dv_prev", np.zeros(P, np.float32))):
+        o.set_field(name, val)
+    o.stage("filter")
+    want = 0.1 if via == "depth" else 0.3
+    for name in ("du_l", "dv_l", "dw_l"):
+        assert abs(float(o.get_field(name)[v0 + u0 * rows]) - want) < 1e-5, name
+
+
 def test_upsampling_of_a_constant_field():
     o = Oracle(30, 16)                    # levels 15x20 and 30x40
     rows, cols = 30, 40
