@@ -6,8 +6,9 @@ the plain solver's by more than 5%.  Applying that rule needs the plain solve of
 costs more than the learned path saves.  This module replaces it with a rule computed from the
 learned solve alone, with constants taken from training pairs only:
 
-  * the finest level's iteration budget: the smallest multiple of 4 (one blocked launch) above
-    the cut at which at least ``target`` of the training pairs meet the plain-referenced rule.
+  * the finest level's iteration budget: the smallest budget, in steps of ``step`` iterations
+    from the cut (4 by default, one blocked launch), at which at least ``target`` of the training
+    pairs meet the plain-referenced rule.
     Per pair, the energy ratio to the plain solve has a spread of about 14% (log scale) that no
     quantity of the learned solve predicts (the two paths linearise the data terms around
     different coarse solutions), so the budget, not a per-pair test, is what brings the learned
@@ -57,10 +58,11 @@ def _plain_rule(e, r, pe, pr):
     return (e <= (1 + PLAIN_TOL) * pe) & (r <= (1 + PLAIN_TOL) * pr)
 
 
-def calibrate(net, n_pairs=2048, factor=2.5, target=0.95, max_finest=96, chunk=256, log=_log):
-    """the rule's constants from training pairs (see the module text)"""
+def calibrate(net, n_pairs=2048, factor=2.5, target=0.95, max_finest=96, step=4, chunk=256, log=_log):
+    """the rule's constants from training pairs (see the module text); finest-level budgets are
+    tried in steps of `step` iterations from the cut"""
     cut = cut_schedule(ROWS, factor)
-    budgets = list(range(cut[-1], max_finest + 1, 4))
+    budgets = list(range(cut[-1], max_finest + 1, step))
     plain, learned = Batch([ROWS] * chunk), Batch([ROWS] * chunk, iters=[cut] * chunk)
     learned.set_learned(net)
     allp = list(range(chunk))
@@ -77,7 +79,7 @@ def calibrate(net, n_pairs=2048, factor=2.5, target=0.95, max_finest=96, chunk=2
         r_ratio.append(pr / p0)
         for k, b in enumerate(budgets):
             if k:
-                learned.continue_finest(allp, 4)      # equals a solve whose finest level ran b
+                learned.continue_finest(allp, step)   # equals a solve whose finest level ran b
             e, r, _ = learned.quality()
             passed[b].append(_plain_rule(e, r, pe, pr))
         log(f"calibrate: {done + chunk} of {n_pairs} training pairs")
@@ -169,6 +171,7 @@ def main(argv=None):
     ap.add_argument("--weights", default=os.path.join(here, "weights", "warmstart_qvga.bin"))
     ap.add_argument("--factor", type=float, default=2.5)
     ap.add_argument("--target", type=float, default=0.95)
+    ap.add_argument("--step", type=int, default=4, help="finest-level budgets tried, in iterations")
     ap.add_argument("--train-pairs", type=int, default=2048)
     ap.add_argument("--eval-pairs", type=int, default=2048)
     ap.add_argument("--out", default=os.path.join(here, "weights", "warmstart_qvga_rule.json"))
@@ -176,7 +179,7 @@ def main(argv=None):
     net = Net(max_batch=256, h=60, w=80)
     net.load(a.weights)
     t0 = time.time()
-    rule = calibrate(net, a.train_pairs, a.factor, a.target)
+    rule = calibrate(net, a.train_pairs, a.factor, a.target, step=a.step)
     t1 = time.time()
     rule["heldout"] = evaluate(net, rule, a.eval_pairs)
     rule["weights"] = os.path.basename(a.weights)
