@@ -894,7 +894,7 @@ int pdf_batch_autotune(pdf_batch *b, const char *cache_path, int *shape_out, int
     int best_shape = b->tb_shape, best_k = b->tb_k, best_tile = b->setup_tile;
     float best = 1e30f;
     int rc = PDF_OK;
-    const int tile0 = b->setup_tile;
+    const int shape0 = b->tb_shape, k0 = b->tb_k, tile0 = b->setup_tile;
     for (int shape : kShapes) {
         for (int k : kKs) {
             float ms = 0.f;
@@ -912,7 +912,13 @@ int pdf_batch_autotune(pdf_batch *b, const char *cache_path, int *shape_out, int
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (rc) return rc;
+    if (rc) {
+        // a failed search leaves the configuration the batch had before it
+        (void)pdf_batch_set_tb_shape(b, shape0);
+        (void)pdf_batch_set_iteration_kernel(b, k0, b->tb_min_pixels, b->level_kernel ? 1 : 0);
+        (void)pdf_batch_set_setup_tile(b, tile0);
+        return rc;
+    }
     rc = pdf_batch_set_tb_shape(b, best_shape);
     if (!rc) rc = pdf_batch_set_iteration_kernel(b, best_k, b->tb_min_pixels, b->level_kernel ? 1 : 0);
     if (!rc) rc = pdf_batch_set_setup_tile(b, best_tile);
