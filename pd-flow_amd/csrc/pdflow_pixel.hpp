@@ -57,12 +57,17 @@ __device__ __forceinline__ float div_half_or_one(float n, float d)
     return n / d;
 }
 
-// 1.f / sqrtf(x) for x >= 1 (1/+inf = 0).  The sequence is evaluated for every x and the
-// result selected, so that callers stay free of branches.
+// 1.f / sqrtf(x) for x >= 1 (1/+inf = 0): the reciprocal of rcp_rn with its residual taken on
+// min(s, FLT_MAX).  For finite x, s = sqrtf(x) < 2^64 is unchanged by the min; for x = +inf the
+// estimate of 1/s is +0 and the Newton step on FLT_MAX leaves it +0, where s itself would give
+// inf * 0 = NaN.  s >= 1, so the min is taken on the bit patterns (one integer min, which needs
+// no NaN canonicalisation).  No branch and no select.
 __device__ __forceinline__ float inv_sqrt_rn_ge1(float x)
 {
-    const float m = rcp_rn_ge1(sqrt_rn_ge1(x));
-    return x < __builtin_inff() ? m : 0.f;
+    const float s = sqrt_rn_ge1(x);
+    const float r0 = __builtin_amdgcn_rcpf(s);
+    const float sc = __uint_as_float(min(__float_as_uint(s), 0x7f7fffffu));
+    return __builtin_fmaf(__builtin_fmaf(-sc, r0, 1.f), r0, r0);
 }
 
 
@@ -108,9 +113,9 @@ __device__ __forceinline__ void project_if_any(float &a, float &b)
 
 __device__ __forceinline__ float clamp_pd(float pd)
 {
-    // .cu:837-846
-    if (fabsf(pd) > 1.f) return (pd > 1.f) ? 1.f : -1.f;
-    return pd;
+    // .cu:837-846: beyond the unit interval the reference returns 1 for pd > 1 and -1 for
+    // pd < -1, i.e. 1 with pd's sign (|pd| > 1 excludes NaN)
+    return fabsf(pd) > 1.f ? __builtin_copysignf(1.f, pd) : pd;
 }
 
 __device__ __forceinline__ float div_axis(float r0, float p0, float rm, float pm, bool first, bool last)

@@ -14,6 +14,9 @@ to set beside bench.py's HIP-event average.
 ``l2``: the L2 hit rate of those launches from one ``--pmc TCC_HIT_sum TCC_MISS_sum`` run of the
 same command.
 
+``counters``: per-launch averages of every counter in one or more ``--pmc`` runs (the SQ
+instruction and wave-state counters) over the same launches, by kernel instance and grid.
+
 ``kernels``: every dispatch of a ``--kernel-trace`` CSV grouped by kernel and grid, with totals
 divided by ``--per`` (the number of solves or pairs traced), and the average of the last
 ``--iteration-launches`` dispatches of the single-pair iteration kernels.
@@ -115,6 +118,35 @@ def l2(a):
     print(f"L2 hit rate over the {a.launches} launches: {th / (th + tm):.3f}")
 
 
+def counters(a):
+    """per-launch averages of every counter of one or more --pmc runs over the last --launches
+    kb_pd_tb dispatches of each, by kernel instance and grid"""
+    table = collections.OrderedDict()
+    names = []
+    for path in a.csv:
+        rows = _sorted_dispatches(_rows(path))
+        by = collections.OrderedDict()
+        for r in rows:
+            d = by.setdefault(int(r["Dispatch_Id"]), {"name": r["Kernel_Name"], "grid": int(r["Grid_Size"])})
+            d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        tb = [d for d in by.values() if "kb_pd_tb" in d["name"]][-a.launches:]
+        if len(tb) != a.launches:
+            raise SystemExit(f"{path}: fewer kb_pd_tb dispatches than --launches")
+        for d in tb:
+            k = (d["name"].split("kb_pd_tb")[1].split("(")[0], d["grid"])
+            for c, v in d.items():
+                if c in ("name", "grid"):
+                    continue
+                if c not in names:
+                    names.append(c)
+                table.setdefault(k, {}).setdefault(c, []).append(v)
+    print("| instance | grid (threads) | " + " | ".join(names) + " |")
+    print("|---|---|" + "---|" * len(names))
+    for (name, grid), cs in table.items():
+        cells = [f"{sum(cs[c]) / len(cs[c]):.4g}" if c in cs else "" for c in names]
+        print(f"| kb_pd_tb{name} | {grid} | " + " | ".join(cells) + " |")
+
+
 def trace(a):
     rows = sorted(_rows(a.trace), key=lambda r: int(r["Start_Timestamp"]))
     tb = [r for r in rows if "kb_pd_tb" in r["Kernel_Name"]][-a.launches:]
@@ -190,11 +222,14 @@ def main(argv=None):
     t.add_argument("--copy-bytes", type=int, required=True)
     t.add_argument("--launches", type=int, required=True)
     t.add_argument("--workload", required=True)
+    c = sub.add_parser("counters")
+    c.add_argument("csv", nargs="+")
+    c.add_argument("--launches", type=int, required=True)
     r = sub.add_parser("trace")
     r.add_argument("trace")
     r.add_argument("--launches", type=int, required=True)
     a = ap.parse_args(argv)
-    {"traffic": traffic, "trace": trace, "kernels": kernels, "l2": l2}[a.cmd](a)
+    {"traffic": traffic, "trace": trace, "kernels": kernels, "l2": l2, "counters": counters}[a.cmd](a)
 
 
 if __name__ == "__main__":
