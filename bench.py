@@ -505,8 +505,10 @@ def run_learned(args):
 
     def run_steps(k, batches, finish):
         """k steps, each a whole batch (solve, rule, continuation, fallback).  Pipelined: the next
-        batch's solve is launched on its own stream before this batch's finish, whose host
-        synchronisations wait on this batch's stream only."""
+        batch's solve is queued on its own stream before this batch's finish, whose host
+        synchronisations wait on this batch's stream only.  That solve waits on the GPU for this
+        batch's solve (wait_for right behind it), so the solves run one at a time at full width and
+        only the finish overlaps the next solve."""
         graph = not args.no_graph
         nb = len(batches)
         batches[0].solve(graph=graph)
@@ -514,6 +516,7 @@ def run_learned(args):
         for s in range(k):
             cur = batches[s % nb]
             if nb > 1 and s + 1 < k:
+                batches[(s + 1) % nb].wait_for(cur)
                 batches[(s + 1) % nb].solve(graph=graph)
             flags = finish(cur)[1]
             if nb == 1 and s + 1 < k:
@@ -545,15 +548,17 @@ def run_learned(args):
             out after its finish"""
             nb = len(batches)
 
-            def start(b):
+            def start(b, after=None):
                 b.upload_inputs_async(src.array)
+                if after is not None:
+                    b.wait_for(after)
                 b.solve(graph=not args.no_graph)
 
             start(batches[0])
             for s in range(k):
                 cur = batches[s % nb]
                 if nb > 1 and s + 1 < k:
-                    start(batches[(s + 1) % nb])
+                    start(batches[(s + 1) % nb], after=cur)
                 finish(cur)
                 cur.download_flows_async(dsts[s % nb].array)
                 if nb == 1 and s + 1 < k:

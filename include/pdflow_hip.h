@@ -271,6 +271,14 @@ int pdf_batch_set_inputs(pdf_batch *b, unsigned pair, const float *I1, const flo
  * stream; use_graph replays a HIP graph captured on first use */
 int pdf_batch_solve(pdf_batch *b, int use_graph);
 int pdf_batch_synchronize(pdf_batch *b);
+/* No counterpart in the reference, which runs one frame pair at a time: orders two batches'
+ * streams without a host wait.  Work queued on b after this call starts only once everything
+ * queued on other before it has finished (an event recorded on other's stream, which b's stream
+ * waits on).  With two batches in flight, queue the next batch's solve after waiting on the
+ * current batch right behind its solve: the solves then run one after the other at full width,
+ * and only the current batch's rule, continuation and re-solves overlap the next solve.  b ==
+ * other does nothing. */
+int pdf_batch_wait(pdf_batch *b, pdf_batch *other);
 /* pair's finest-level flow planes (rows x cols, column-major) on the device / copied out */
 int pdf_batch_flow_device(pdf_batch *b, unsigned pair, const float **dx, const float **dy,
                           const float **dz, unsigned *rows, unsigned *cols);

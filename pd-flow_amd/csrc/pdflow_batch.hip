@@ -117,6 +117,7 @@ struct pdf_batch {
     char *dev_tables = nullptr;
     size_t dev_tables_bytes = 0;
     hipGraphExec_t graph = nullptr;
+    hipEvent_t order_event = nullptr;   // recorded by pdf_batch_wait on this batch's stream
     // HIP-event timing of the iteration launches (eager solves only)
     bool timing = false;
     bool timing_tb_only = false;     // time only the temporally blocked iteration launches
@@ -768,6 +769,7 @@ int pdf_batch_destroy(pdf_batch *b)
     (void)hipStreamSynchronize(b->stream);
     for (auto &ev : b->events) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->order_event) (void)hipEventDestroy(b->order_event);
     if (b->dev_tables) (void)hipFree(b->dev_tables);
     if (b->scratch) (void)hipFree(b->scratch);
     if (b->learn_x) (void)hipFree(b->learn_x);
@@ -1020,6 +1022,17 @@ int pdf_batch_synchronize(pdf_batch *b)
     if (!b) return PDF_ERR_ARG;
     BHIP(b, hipStreamSynchronize(b->stream));
     return finish_timing(b);
+}
+
+int pdf_batch_wait(pdf_batch *b, pdf_batch *other)
+{
+    if (!b || !other) return PDF_ERR_ARG;
+    if (b == other) return PDF_OK;
+    if (!other->order_event)
+        BHIP(other, hipEventCreateWithFlags(&other->order_event, hipEventDisableTiming));
+    BHIP(other, hipEventRecord(other->order_event, other->stream));
+    BHIP(b, hipStreamWaitEvent(b->stream, other->order_event, 0));
+    return PDF_OK;
 }
 
 int pdf_batch_enable_kernel_timing(pdf_batch *b, int enable)

@@ -116,6 +116,7 @@ def hip_library():
         "pdf_batch_set_inputs": ([vp, c_uint, fp, fp, fp, fp], c_int),
         "pdf_batch_solve": ([vp, c_int], c_int),
         "pdf_batch_synchronize": ([vp], c_int),
+        "pdf_batch_wait": ([vp, vp], c_int),
         "pdf_batch_flow_device": ([vp, c_uint, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), up_, up_], c_int),
         "pdf_batch_copy_flow": ([vp, c_uint, fp, fp, fp], c_int),
         "pdf_batch_upload_inputs_async": ([vp, fp, ctypes.c_size_t], c_int),
@@ -503,6 +504,11 @@ class Batch:
 
     def synchronize(self):
         self._chk(self.lib.pdf_batch_synchronize(self.ctx))
+
+    def wait_for(self, other):
+        """work queued on this batch from now on starts only once everything queued on `other`
+        so far has finished (no host wait)"""
+        self._chk(self.lib.pdf_batch_wait(self.ctx, other.ctx))
 
     def upload_inputs_async(self, host):
         """queues the copy of every pair's I1, Z1, I2, Z2 (pair after pair) from the float32 array

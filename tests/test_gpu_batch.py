@@ -321,3 +321,35 @@ def test_async_copies_when_frames_are_larger_than_the_solve(gpu_available):
         off += 3 * P
     src.free()
     dst.free()
+
+
+def test_two_batches_in_flight_ordered_by_wait(gpu_available):
+    """pdf_batch_wait: two batches solved alternately, each solve queued behind the other batch's
+    (the pipelined pattern of the benchmark, with a rule-like metrics pass and a re-solve on the
+    current batch overlapping the next solve), give every pair's single-pair flow; waiting on
+    itself is a no-op and a null batch is refused."""
+    import ctypes
+    rows = [240, 120, 60]
+    say(f"two batches ordered by pdf_batch_wait, {rows}")
+    pairs = [single(r, 1500 + i) for i, r in enumerate(rows)]
+    a, b = Batch(rows), Batch(rows)
+    for i, (frames, _) in enumerate(pairs):
+        a.set_inputs(i, *frames)
+        b.set_inputs(i, *frames)
+    a.wait_for(a)
+    cur, nxt = a, b
+    cur.solve(graph=True)
+    for step in range(4):
+        nxt.wait_for(cur)
+        nxt.solve(graph=True)
+        cur.quality()                     # host waits on cur's stream only
+        cur.resolve_plain([1])
+        cur.synchronize()
+        for i, (_, ref) in enumerate(pairs):
+            for k in range(3):
+                np.testing.assert_array_equal(cur.flow(i)[k], ref[k], err_msg=f"step {step} pair {i} plane {k}")
+        cur, nxt = nxt, cur
+    cur.synchronize()
+    lib = a.lib
+    assert lib.pdf_batch_wait(None, a.ctx) != 0
+    assert lib.pdf_batch_wait(a.ctx, None) != 0
