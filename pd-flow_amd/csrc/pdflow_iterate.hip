@@ -308,11 +308,14 @@ __device__ __forceinline__ void dual_px(Px &q, float nU, float nV, float nW, flo
 // rjv = rj there.  In the first column (row) the caller passes +0 for the weight and the duals
 // (an LDS slot that holds +0): the product is +0 and f - (+0) is f bit for bit, the
 // reference's first-column term.
+// EDGE = false: the caller knows that no pixel of the wave lies in the image's last column or
+// row, where the divergence takes its one-sided form, so that case is left out
+template <bool EDGE = true>
 __device__ __forceinline__ void primal_px(Px &q, int rows, int cols, float riu, float puu,
                                           float pvu, float pwu, float rjv, float puv, float pvv,
                                           float pwv, float li, float ld)
 {
-    const bool ul = q.gu == cols - 1, vl = q.gv == rows - 1;
+    const bool ul = EDGE && q.gu == cols - 1, vl = EDGE && q.gv == rows - 1;
     float divu = div_axis_last(q.ri, q.p[0], riu, puu, ul);
     float divv = div_axis_last(q.ri, q.p[2], riu, pvu, ul);
     float divw = div_axis_last(q.ri, q.p[4], riu, pwu, ul);
@@ -447,12 +450,23 @@ __device__ __forceinline__ void tb_iterate(const PDArgs &a, TBLds<RV, RU> &s, co
 #pragma unroll
         for (int k = 0; k < 6; k++) s.p[k][idx] = q.p[k];
     };
+    // most waves hold no pixel of the image's last column or row: they take the primal step
+    // without that case's selects (a wave-uniform branch)
+    bool edge = false;
+    auto find_edge = [&](Px &q, int) __attribute__((always_inline)) {
+        edge = edge || __builtin_amdgcn_ballot_w64(q.gu == cols - 1 || q.gv == rows - 1) != 0;
+    };
+    PD_EACH(find_edge);
     auto primal = [&](Px &q, int j) __attribute__((always_inline)) {
         const int lu = 2 * slot + j, idx = lu * RV + lv;
         const int ju = lu > 0 && q.gu > 0 ? idx - RV : s.kZero;
         const int jv = lv > 0 && q.gv > 0 ? idx - 1 : s.kZero;
-        primal_px(q, rows, cols, s.ri[ju], s.p[0][ju], s.p[2][ju], s.p[4][ju], s.rj[jv], s.p[1][jv],
-                  s.p[3][jv], s.p[5][jv], li, ld);
+        if (edge)
+            primal_px<true>(q, rows, cols, s.ri[ju], s.p[0][ju], s.p[2][ju], s.p[4][ju], s.rj[jv],
+                            s.p[1][jv], s.p[3][jv], s.p[5][jv], li, ld);
+        else
+            primal_px<false>(q, rows, cols, s.ri[ju], s.p[0][ju], s.p[2][ju], s.p[4][ju], s.rj[jv],
+                             s.p[1][jv], s.p[3][jv], s.p[5][jv], li, ld);
     };
     for (int it = 0; it < K; it++) {
         if (reach >= it) PD_EACH(put_bar);
