@@ -147,7 +147,10 @@ bool pd_tb_supported(int k);
 // iterations each (k = 1: the single-iteration kernel): launches of K, with the remainder (a
 // remainder of 1 joined to the last K) as one supported size or two balanced ones (7 = 4 + 3),
 // so that no level of 2 or more iterations ends with single-iteration launches, which cost
-// about twice a blocked iteration.
+// about twice a blocked iteration.  Where K + 1 is supported and there are at least r launches,
+// a remainder of r >= 2 is spread over the last r launches instead (34 = 6 x 4 + 2 x 5): a
+// short launch moves the same bytes as a full one, and one more iteration per launch costs
+// little.
 // for_each_tb_chunk calls f(k) for each launch of that split in order, allocates nothing, and
 // stops at the first call that returns nonzero, returning that value (0 if none does).
 template <class F>
@@ -159,6 +162,13 @@ inline int for_each_tb_chunk(unsigned n, int K, F &&f)
         return 0;
     }
     unsigned q = n / (unsigned)K, r = n % (unsigned)K;
+    if (r >= 2 && r <= q && pd_tb_supported(K + 1)) {
+        for (unsigned i = 0; i < q - r; i++)
+            if (int rc = f(K)) return rc;
+        for (unsigned i = 0; i < r; i++)
+            if (int rc = f(K + 1)) return rc;
+        return 0;
+    }
     if (r == 1 && q > 0) {
         q--;
         r += (unsigned)K;
